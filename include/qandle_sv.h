@@ -1,0 +1,103 @@
+/*
+ * qandle_sv.h -- C-ABI of the MI355X (gfx950) state-vector engine behind the
+ * qandle_amd drop-in.  One shared library, libqandle_sv.so, exports these.
+ *
+ * Conventions for every entry point
+ *   - Amplitudes are complex64 stored as interleaved float pairs (re, im).
+ *     A state batch is a dense (B, 2^n) row-major array: sample b, amplitude
+ *     index i at element b * 2^n + i.  qandle qubit q is amplitude bit n-1-q;
+ *     every entry point here speaks of BIT POSITIONS, the Python host converts.
+ *   - All pointers are device pointers unless a parameter says "host".
+ *   - `stream` is a hipStream_t passed as void*; work is enqueued on it and the
+ *     call returns without synchronising.
+ *   - Return value: QS_OK (0) or a QS_E* code; no exception crosses the ABI.
+ *     qs_last_hip_error() gives the HIP error behind QS_EHIP.
+ *   - No entry point allocates device memory.  Entry points that need scratch
+ *     take a caller-provided `work` buffer whose size is given by the matching
+ *     *_work_bytes query.
+ *
+ * Reference citations are file:line into gstenzel/qandle v0.1.8.
+ */
+#ifndef QANDLE_SV_H
+#define QANDLE_SV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  QS_OK = 0,
+  QS_EINVAL = 1,       /* bad argument (shape, bit position, null pointer) */
+  QS_EHIP = 2,         /* a HIP runtime call failed, see qs_last_hip_error() */
+  QS_EWORKSPACE = 3,   /* caller workspace too small */
+  QS_EUNSUPPORTED = 4  /* valid request this build does not handle */
+};
+
+/* Rotation kinds and Pauli generators: R_P(phi) = exp(-i phi P / 2). */
+enum { QS_RX = 0, QS_RY = 1, QS_RZ = 2 };
+
+int qs_abi_version(void);
+int qs_last_hip_error(void);
+const char *qs_status_string(int status);
+
+/* ---------------------------------------------------------------------------
+ * Gate coefficients.  cs[2k] = cos(phi[k]/2), cs[2k+1] = sin(phi[k]/2).
+ * Replaces the per-call matrix rebuild `_a*a_op(t) + _b*b_op(t)`, t = phi/2
+ * (src/qandle/operators.py:265-275): only the two scalars are ever needed.
+ * ------------------------------------------------------------------------- */
+int qs_half_angle_coeffs(float *cs, const float *phi, int64_t count, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Plain per-gate path (one HBM pass per gate).
+ * ------------------------------------------------------------------------- */
+
+/* AngleEmbedding: out[b] = (x)_j R_rot(x[b, j]) |0..0> over the embedded bit
+ * positions bits[0..d) (host array); unembedded bits stay |0>.
+ * x is (B, d) float32 with row stride x_stride (0 = one row shared by all B).
+ * Replaces AngleEmbeddingBuilt.forward (src/qandle/embeddings.py:148-164). */
+int qs_embed_angle(float *out, const float *x, int64_t B, int n,
+                   const int32_t *bits, int d, int rot, int64_t x_stride,
+                   void *stream);
+
+/* One rotation on bit position `bit`: out = U psi (out may equal in).
+ * Coefficients for sample b are cs[2*(b*cs_stride)], cs_stride 0 = shared,
+ * 1 = one (cos, sin) pair per sample.  inverse != 0 applies U^dagger.
+ * Replaces BuiltParametrizedOperator.forward `state @ M` / the batched bmm
+ * (src/qandle/operators.py:289-298). */
+int qs_rot(float *out, const float *in, int64_t B, int n, int bit, int kind,
+           const float *cs, int64_t cs_stride, int inverse, void *stream);
+
+/* CNOT as an exact permutation: amplitudes i and i ^ (1<<tbit) exchange when
+ * bit cbit of i is set.  out may equal in (in-place swap).
+ * Replaces BuiltCNOT.forward `state @ _M` (src/qandle/operators.py:547-558). */
+int qs_cnot(float *out, const float *in, int64_t B, int n, int cbit, int tbit,
+            void *stream);
+
+/* MeasureProbability: out[b, q] = sum_{i: bit n-1-q of i = 0} |psi_b[i]|^2,
+ * float32 (B, n).  Replaces MeasureProbabilityBuilt.forward
+ * (src/qandle/measurements.py:120-123).  Deterministic two-stage reduction. */
+int64_t qs_measure_prob_work_bytes(int64_t B, int n);
+int qs_measure_prob(float *out, const float *psi, int64_t B, int n, void *work,
+                    int64_t work_bytes, void *stream);
+
+/* Backward of MeasureProbability: lam[b, i] = 2 psi[b, i] * sum_q w[b, q] [bit n-1-q of i = 0]
+ * (torch's dL/dRe + i dL/dIm for L = sum w * P).  w is (B, n) float32 with
+ * row stride w_stride (0 = shared). */
+int qs_measure_prob_backward(float *lam, const float *psi, const float *w,
+                             int64_t w_stride, int64_t B, int n, void *stream);
+
+/* out[b * d + j] = Im <lam_b | P_j | psi_b> for the Pauli `pauli` on bit
+ * position bits[j] (host array, d <= 64).  Deterministic; used for the
+ * rotation and embedding gradients of the plain path, where
+ * dL/dphi = 1/2 Im <lam | P | psi> at the gate output. */
+int64_t qs_pauli_im_work_bytes(int64_t B, int n, int d);
+int qs_pauli_im(float *out, const float *lam, const float *psi, int64_t B,
+                int n, const int32_t *bits, int d, int pauli, void *work,
+                int64_t work_bytes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QANDLE_SV_H */

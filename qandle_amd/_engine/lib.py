@@ -1,0 +1,92 @@
+"""ctypes binding of libqandle_sv.so (the C-ABI declared in include/qandle_sv.h).
+
+The library is built in-tree by ``qandle_amd/csrc/build.py`` (hipcc for
+gfx950).  There is no fallback: if the library is missing, or a tensor is not
+on a HIP device, the call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch
+
+from ..errors import EngineError
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "_lib", "libqandle_sv.so"))
+
+_lock = threading.Lock()
+_lib = None
+
+P = ctypes.c_void_p
+I32 = ctypes.c_int
+I64 = ctypes.c_int64
+
+# name -> (restype, argtypes); must match include/qandle_sv.h
+SIGNATURES = {
+    "qs_abi_version": (I32, []),
+    "qs_last_hip_error": (I32, []),
+    "qs_status_string": (ctypes.c_char_p, [I32]),
+    "qs_half_angle_coeffs": (I32, [P, P, I64, P]),
+    "qs_embed_angle": (I32, [P, P, I64, I32, P, I32, I32, I64, P]),
+    "qs_rot": (I32, [P, P, I64, I32, I32, I32, P, I64, I32, P]),
+    "qs_cnot": (I32, [P, P, I64, I32, I32, I32, P]),
+    "qs_measure_prob_work_bytes": (I64, [I64, I32]),
+    "qs_measure_prob": (I32, [P, P, I64, I32, P, I64, P]),
+    "qs_measure_prob_backward": (I32, [P, P, P, I64, I64, I32, P]),
+    "qs_pauli_im_work_bytes": (I64, [I64, I32, I32]),
+    "qs_pauli_im": (I32, [P, P, P, I64, I32, P, I32, I32, P, I64, P]),
+}
+
+
+def load():
+    """Load (once) and return the engine library; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise EngineError(
+                f"HIP engine library not found at {LIB_PATH}; build it with "
+                "`python -c 'import __graft_entry__ as g; g.build()'` (hipcc, gfx950)"
+            )
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        lib = load()
+        msg = lib.qs_status_string(status).decode()
+        extra = f" (hip error {lib.qs_last_hip_error()})" if status == 2 else ""
+        raise EngineError(f"{what}: {msg}{extra}")
+
+
+def stream_of(t: torch.Tensor) -> int:
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+def require_device(t: torch.Tensor, what: str = "tensor") -> None:
+    if t.device.type != "cuda":
+        raise EngineError(
+            f"qandle_amd runs on the MI355X HIP engine only; {what} is on {t.device}. "
+            "Move the circuit and its inputs to a HIP device (module.to('cuda'))."
+        )
+
+
+def ptr(t: torch.Tensor) -> int:
+    return t.data_ptr()
+
+
+def i32_array(values):
+    values = [int(v) for v in values]
+    return (ctypes.c_int32 * max(1, len(values)))(*values)

@@ -1,0 +1,13 @@
+"""Module-level configuration (``src/qandle/config.py:1-3``).
+
+``DEFAULT_MAPPING`` is read when an unnamed rotation is constructed
+(``src/qandle/operators.py:168``), exactly as in the reference.
+"""
+from . import qw_map
+
+DEFAULT_MAPPING = qw_map.tanh
+
+# Engine knobs (no counterpart in the reference).
+# Gate runs wider than this many register/LDS bits are split across passes.
+FUSED = True            # Circuit.forward uses the fused pass engine
+CHECKPOINT_LAYERS = 5   # adjoint backward restores psi from a checkpoint every k layers

@@ -1,0 +1,31 @@
+// Word format of a fused pass program (shared by the host planner and the kernels).
+//
+// A pass is a stream of int32 words in device memory:
+//
+//   for each sub-pass:
+//     word 0            : number of ops in this sub-pass
+//     words 1 .. R      : the R tile-local bit indices held in registers
+//                         (R is the kernel's compile-time register-bit count)
+//     then 4 words per op: opcode, a, b, coef
+//
+// Opcodes (low 8 bits of word 0 of an op; bit 8 = per-sample coefficient,
+// i.e. the coefficient index is coef + sample instead of coef):
+//   QS_OP_RX, QS_OP_RY, QS_OP_RZ  rotation on register bit a
+//   QS_OP_RZ_UNIFORM              RZ on global bit position a that is not a
+//                                 register bit: a per-thread phase
+//   QS_OP_CNOT                    target register bit a, control register bit b
+//   QS_OP_CNOT_UNIFORM            target register bit a, control = global bit
+//                                 position b that is not a register bit
+// Coefficients are float2 (cos(phi/2), sin(phi/2)) entries of the `cs` table.
+#pragma once
+
+#define QS_OP_RX 0
+#define QS_OP_RY 1
+#define QS_OP_RZ 2
+#define QS_OP_RZ_UNIFORM 3
+#define QS_OP_CNOT 4
+#define QS_OP_CNOT_UNIFORM 5
+#define QS_OP_PER_SAMPLE 0x100
+
+#define QS_MAX_LOCAL_BITS 16
+#define QS_MAX_QUBITS 40

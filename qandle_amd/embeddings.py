@@ -1,0 +1,98 @@
+"""AngleEmbedding (``src/qandle/embeddings.py:114-214``).
+
+The reference builds d dense (2^n x 2^n) rotation matrices per call and
+multiplies them together (an O(B d 8^n) chain, ``:148-164``).  The product
+state (x)_j R(x_j)|0> is separable, so the engine writes it directly:
+``qs_embed_angle`` in the plain path, or as the generator of the first fused
+pass inside a Circuit.
+"""
+from __future__ import annotations
+
+import typing
+
+import torch
+
+from . import operators as op
+from . import utils
+from ._engine import plain
+
+__all__ = ["AngleEmbedding", "AngleEmbeddingBuilt", "InputOperator", "InputOperatorBuilt"]
+
+
+class InputOperator(op.UnbuiltOperator):
+    pass
+
+
+class InputOperatorBuilt(op.BuiltOperator):
+    def decompose(self):
+        raise NotImplementedError(f"Decomposing {self.__class__} is not yet supported")
+
+    def to_matrix(self, **kwargs):
+        raise ValueError("Input operators do not have a matrix representation")
+
+
+_ROT_KIND = {"RX": 0, "RY": 1, "RZ": 2}
+
+
+class AngleEmbeddingBuilt(InputOperatorBuilt):
+    """Ignores the incoming state and returns (x)_q R(x_q / 2-angle)|0..0>.
+
+    Input ``kwargs[name]`` has shape (d,) or (B, d); output (2^n,) or (B, 2^n)
+    complex64.  The half-angle is x/2 with no remapping (``:150-151``).
+    """
+
+    def __init__(self, name: str, num_qubits: int, qubits: typing.List[int], rotation: str):
+        super().__init__()
+        self.num_qubits = num_qubits
+        self.qubits = list(qubits)
+        self.named = True
+        self.name = name
+        self.rotation = rotation
+        self.rots = utils.parse_rot(rotation)
+        self.kind = _ROT_KIND[self.rots.__name__]
+        self.bits = [op.qubit_bit(num_qubits, q) for q in self.qubits]
+        # The reference builds one throw-away rotation per embedded qubit
+        # (``src/qandle/embeddings.py:140-146``); each consumes torch.rand(1) for
+        # its unused theta.  Consume the same draws so that seeded parameter
+        # initialisation of later layers matches the reference.
+        for _ in self.qubits:
+            torch.rand(1)
+
+    def forward(self, state=None, **kwargs):
+        x = kwargs[self.name]
+        from ._engine.lib import require_device
+
+        require_device(x, f"input '{self.name}'")
+        if x.shape[-1] != len(self.qubits):
+            raise RuntimeError(f"AngleEmbedding '{self.name}' expects {len(self.qubits)} features, got {x.shape[-1]}")
+        if x.dim() == 1:
+            return plain.EmbedFn.apply(x.unsqueeze(0), self.num_qubits, self.bits, self.kind).squeeze(0)
+        if x.dim() == 2:
+            return plain.EmbedFn.apply(x, self.num_qubits, self.bits, self.kind)
+        lead = x.shape[:-1]
+        out = plain.EmbedFn.apply(x.reshape(-1, x.shape[-1]), self.num_qubits, self.bits, self.kind)
+        return out.reshape(lead + (2 ** self.num_qubits,))
+
+    def __str__(self) -> str:
+        return f"AngleEmbedding_{self.rotation}"
+
+    def decompose(self):
+        """The per-qubit named rotations this embedding is equivalent to."""
+        return [self.rots(qubit=w, name=f"{self.name}_{w}") for w in self.qubits]
+
+
+class AngleEmbedding(InputOperator):
+    """Angle embedding (``src/qandle/embeddings.py:184-214``); ``name`` is required."""
+
+    def __init__(self, name: str, qubits: typing.Union[typing.List[int], None] = None, rotation="rx"):
+        self.name = name
+        self.named = True
+        self.qubits = qubits
+        self.rotation = rotation
+
+    def build(self, num_qubits: int, **kwargs) -> AngleEmbeddingBuilt:
+        qubits = list(range(num_qubits)) if self.qubits is None else self.qubits
+        return AngleEmbeddingBuilt(name=self.name, qubits=qubits, num_qubits=num_qubits, rotation=self.rotation)
+
+    def __str__(self) -> str:
+        return f"AngleEmbedding_{self.rotation}"

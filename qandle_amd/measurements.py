@@ -1,0 +1,62 @@
+"""MeasureProbability (``src/qandle/measurements.py:94-126``).
+
+P(qubit q = 0) per sample as float32 (B, n).  The reference repeats the state
+n times, masks and reduces (``:120-123``, an n x state temporary); the engine
+runs one deterministic |psi|^2 reduction (``qs_measure_prob``), and inside a
+Circuit fuses it into the last pass.  The reference's squeeze quirks are kept:
+an unbatched state gives (n,), and so does a batched state with B = 1.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import operators as op
+from ._engine import plain
+
+__all__ = ["MeasureProbability", "MeasureProbabilityBuilt", "BuiltMeasurement", "UnbuiltMeasurement"]
+
+
+def reference_shape(probs_bn: torch.Tensor) -> torch.Tensor:
+    """Apply ``.squeeze().transpose(-1, 0)`` to the (n, B) layout, as the reference does."""
+    return probs_bn.transpose(0, 1).squeeze().transpose(-1, 0)
+
+
+class BuiltMeasurement(op.BuiltOperator):
+    def __str__(self) -> str:
+        return "M"
+
+    def to_matrix(self, **kwargs):
+        return torch.eye(2 ** self.num_qubits)
+
+
+class UnbuiltMeasurement(op.UnbuiltOperator):
+    def __str__(self) -> str:
+        return "M"
+
+    def build(self, num_qubits: int, **kwargs) -> BuiltMeasurement:
+        return BUILT_CLASS_RELATION[self.__class__](num_qubits=num_qubits, **kwargs)
+
+
+class MeasureProbability(UnbuiltMeasurement):
+    r"""Probability of each qubit being in :math:`|0\rangle`."""
+
+
+class MeasureProbabilityBuilt(BuiltMeasurement):
+    def __init__(self, num_qubits: int):
+        super().__init__()
+        self.num_qubits = num_qubits
+        self.qubits = list(range(num_qubits))
+
+    def forward(self, state: torch.Tensor):
+        from ._engine.lib import require_device
+
+        require_device(state, "state")
+        n = self.num_qubits
+        if state.shape[-1] != 2 ** n:
+            raise RuntimeError(f"state has {state.shape[-1]} amplitudes, expected {2 ** n}")
+        st = state.reshape(-1, 2 ** n)
+        probs = plain.MeasureProbFn.apply(st, n)
+        return reference_shape(probs)
+
+
+BUILT_CLASS_RELATION = {MeasureProbability: MeasureProbabilityBuilt}

@@ -1,0 +1,329 @@
+"""Operators of the drop-in API (``src/qandle/operators.py``).
+
+Same class names, constructor contracts, ``build()`` protocol, ``named``
+dispatch flag, ``state_dict`` keys (``...theta``) and registry
+(``BUILT_CLASS_RELATION`` with its ``.T`` inverse) as the reference.  What
+changed is underneath: no operator holds a 2^n x 2^n gate-matrix cache.  A
+rotation is applied by the HIP kernel ``qs_rot`` from two scalars
+(cos(phi/2), sin(phi/2)); CNOT is the exact permutation ``qs_cnot``.  Inside a
+``Circuit`` the operators are not called one by one at all: the circuit lowers
+them to a program for the fused engine.
+"""
+from __future__ import annotations
+
+import abc
+import typing
+
+import torch
+
+from . import config
+from . import errors
+from . import qw_map
+from . import utils
+from ._engine import plain
+
+__all__ = [
+    "Operator",
+    "RX",
+    "RY",
+    "RZ",
+    "CNOT",
+    "BUILT_CLASS_RELATION",
+]
+
+
+class AbstractNoForward(abc.ABCMeta, utils.do_not_implement("forward", "backward")):
+    """Forbids ``forward``/``backward`` on unbuilt operators (``src/qandle/operators.py:31-32``)."""
+
+
+def qubit_bit(num_qubits: int, qubit: int) -> int:
+    """Amplitude-index bit position of qandle qubit ``qubit``.
+
+    qandle orders the Kronecker product with qubit 0 first
+    (``src/qandle/operators.py:284-286``), so qubit q is bit n-1-q
+    (``:549``).  This is the only place the engine converts.
+    """
+    if not 0 <= qubit < num_qubits:
+        raise ValueError(f"qubit {qubit} out of range for {num_qubits} qubits")
+    return num_qubits - 1 - qubit
+
+
+class Operator(abc.ABC):
+    """Everything that can be applied to a state (``src/qandle/operators.py:35-49``)."""
+
+    named = False
+
+    @abc.abstractmethod
+    def __str__(self) -> str:
+        """String representation."""
+
+    def __repr__(self) -> str:
+        return self.__str__()
+
+    def to_qasm(self):
+        raise NotImplementedError("OpenQASM export is not part of the qandle_amd engine")
+
+
+class UnbuiltOperator(Operator, abc.ABC):
+    """Operators that have not been built yet (``src/qandle/operators.py:52-57``)."""
+
+    @abc.abstractmethod
+    def build(self, num_qubits, **kwargs) -> "BuiltOperator":
+        """Converts the operator to a torch.nn.Module for ``num_qubits`` qubits."""
+
+
+class BuiltOperator(Operator, torch.nn.Module, abc.ABC):
+    """Built operators (``src/qandle/operators.py:60-69``)."""
+
+    @abc.abstractmethod
+    def forward(self, state: torch.Tensor) -> torch.Tensor:
+        """Applies the operator to the state."""
+
+    @abc.abstractmethod
+    def to_matrix(self, **kwargs) -> torch.Tensor:
+        """Matrix M with ``state @ M == forward(state)``."""
+
+
+# ----------------------------------------------------------------------------
+# batch handling shared by the single-operator forwards (S5 of the reference:
+# an unbatched state with a batched named input yields a batched output,
+# ``src/qandle/operators.py:290-298``)
+# ----------------------------------------------------------------------------
+
+def _as_batch(state: torch.Tensor, num_qubits: int):
+    if state.shape[-1] != 2 ** num_qubits:
+        raise RuntimeError(
+            f"state has {state.shape[-1]} amplitudes, operator expects {2 ** num_qubits}"
+        )
+    if state.dim() == 1:
+        return state.unsqueeze(0), True
+    if state.dim() == 2:
+        return state, False
+    raise RuntimeError("qandle_amd operators take (2^n,) or (B, 2^n) states")
+
+
+class UnbuiltParametrizedOperator(UnbuiltOperator, metaclass=AbstractNoForward):
+    """Parametrised rotation before build (``src/qandle/operators.py:150-212``)."""
+
+    def __init__(
+        self,
+        qubit: int,
+        theta: typing.Union[float, torch.Tensor, None] = None,
+        name: typing.Union[str, None] = None,
+        **kwargs,
+    ):
+        assert isinstance(qubit, int), "qubit must be an integer"
+        assert qubit >= 0, "qubit must be >= 0"
+        if isinstance(theta, (float, int)):
+            theta = torch.tensor(theta, requires_grad=True, dtype=torch.float)
+        remapping = kwargs.get("remapping", config.DEFAULT_MAPPING)
+        if remapping is None:
+            remapping = qw_map.none
+        self.qubit = qubit
+        self.name = name
+        self.named = name is not None
+        self.theta = theta
+        self.remapping = remapping
+
+    def __str__(self) -> str:
+        if self.name is None:
+            if self.theta is None:
+                return f"{self.__class__.__name__}{self.qubit}"
+            return f"{self.__class__.__name__}{self.qubit} ({self.remapping(self.theta).item():.2f})"
+        return f"{self.__class__.__name__}{self.qubit} ({self.name})"
+
+    def build(self, num_qubits, **kwargs) -> "BuiltParametrizedOperator":
+        return BUILT_CLASS_RELATION[self.__class__](
+            qubit=self.qubit,
+            initialtheta=self.theta,
+            name=self.name,
+            remapping=self.remapping,
+            num_qubits=num_qubits,
+        )
+
+
+class BuiltParametrizedOperator(BuiltOperator, abc.ABC):
+    """Built rotation (``src/qandle/operators.py:215-305``) without the matrix cache.
+
+    ``theta`` is an ``nn.Parameter`` initialised exactly as the reference does
+    (``torch.rand(1)`` when none is given, consumed at build time).
+    """
+
+    KIND = -1  # 0 = X, 1 = Y, 2 = Z (the Pauli generator)
+
+    def __init__(
+        self,
+        qubit: int,
+        remapping: typing.Callable,
+        num_qubits: int,
+        initialtheta: typing.Union[torch.Tensor, None] = None,
+        name: typing.Union[str, None] = None,
+    ):
+        super().__init__()
+        self.qubit = qubit
+        if initialtheta is None:
+            initialtheta = torch.rand(1)
+        self.name = name
+        self.named = name is not None
+        self.theta = torch.nn.Parameter(initialtheta, requires_grad=True)
+        self.remapping = remapping
+        self.num_qubits = num_qubits
+        self.unbuilt_class = BUILT_CLASS_RELATION.T[self.__class__]
+        self.bit = qubit_bit(num_qubits, qubit)
+
+    def __str__(self) -> str:
+        base = f"{self.unbuilt_class.__name__}_{self.qubit}"
+        if self.named:
+            return f"{base} ({self.name})"
+        return f"{base} ({self.remapping(self.theta).item():.2f})"
+
+    def angle(self, **kwargs) -> torch.Tensor:
+        """phi: remap(theta) for an unnamed gate, kwargs[name] (no remap) for a named one
+        (``src/qandle/operators.py:265-271``)."""
+        if self.named:
+            return kwargs[self.name]
+        return self.remapping(self.theta)
+
+    def forward(self, state: torch.Tensor, **kwargs) -> torch.Tensor:
+        phi = self.angle(**kwargs)
+        if not isinstance(phi, torch.Tensor):
+            phi = torch.tensor(phi, dtype=torch.float32)
+        from ._engine.lib import require_device
+
+        require_device(state, "state")
+        phi = phi.to(device=state.device, dtype=torch.float32)
+        if phi.dim() > 1:
+            raise RuntimeError("named rotation inputs must be 0-d or (B,)")
+        if phi.numel() == 1 and phi.dim() == 1 and not self.named:
+            phi = phi.reshape(())
+        st, unbatched = _as_batch(state, self.num_qubits)
+        if phi.dim() == 1:
+            if st.shape[0] == 1 and phi.shape[0] != 1:
+                st = st.expand(phi.shape[0], -1)
+            elif st.shape[0] != phi.shape[0]:
+                raise RuntimeError(
+                    f"batched input of size {phi.shape[0]} does not match state batch {st.shape[0]}"
+                )
+            unbatched = False
+        out = plain.RotFn.apply(st, phi, self.num_qubits, self.bit, self.KIND)
+        return out.squeeze(0) if unbatched else out
+
+    def to_matrix(self, **kwargs) -> torch.Tensor:
+        """Dense ``M = U^T`` expanded to 2^n x 2^n (a debugging aid; never on the hot path)."""
+        phi = self.angle(**kwargs)
+        if not isinstance(phi, torch.Tensor):
+            phi = torch.tensor(phi, dtype=torch.float32)
+        t = phi / 2
+        c, s = torch.cos(t), torch.sin(t)
+        z = torch.zeros_like(c)
+        if self.KIND == 0:
+            u = torch.stack([torch.complex(c, z), torch.complex(z, -s), torch.complex(z, -s), torch.complex(c, z)], -1)
+        elif self.KIND == 1:
+            u = torch.stack([torch.complex(c, z), torch.complex(-s, z), torch.complex(s, z), torch.complex(c, z)], -1)
+        else:
+            u = torch.stack([torch.complex(c, -s), torch.complex(z, z), torch.complex(z, z), torch.complex(c, s)], -1)
+        u = u.reshape(u.shape[:-1] + (2, 2))
+        m = torch.ones((1, 1), dtype=torch.complex64)
+        eye = torch.eye(2, dtype=torch.complex64)
+        for w in range(self.num_qubits):
+            m = torch.kron(m, u if w == self.qubit else eye) if u.dim() == 2 else _bkron(m, u if w == self.qubit else eye)
+        return m.transpose(-1, -2)
+
+
+def _bkron(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    if a.dim() == 2:
+        a = a.unsqueeze(0)
+    if b.dim() == 2:
+        b = b.unsqueeze(0)
+    out = a[:, :, None, :, None] * b[:, None, :, None, :]
+    return out.reshape(out.shape[0], a.shape[1] * b.shape[1], a.shape[2] * b.shape[2])
+
+
+class RX(UnbuiltParametrizedOperator):
+    """Parametrised RX gate, exp(-i phi X / 2) (``src/qandle/operators.py:308-325``)."""
+
+
+class RY(UnbuiltParametrizedOperator):
+    """Parametrised RY gate, exp(-i phi Y / 2) (``src/qandle/operators.py:328-345``)."""
+
+
+class RZ(UnbuiltParametrizedOperator):
+    """Parametrised RZ gate, exp(-i phi Z / 2) (``src/qandle/operators.py:348-365``)."""
+
+
+class BuiltRX(BuiltParametrizedOperator):
+    KIND = 0
+
+
+class BuiltRY(BuiltParametrizedOperator):
+    KIND = 1
+
+
+class BuiltRZ(BuiltParametrizedOperator):
+    KIND = 2
+
+
+class CNOT(UnbuiltOperator):
+    """CNOT gate (``src/qandle/operators.py:398-413``)."""
+
+    def __init__(self, control: int, target: int):
+        assert control != target, "Control and target must be different"
+        self.c = control
+        self.t = target
+
+    def __str__(self) -> str:
+        return f"CNOT {self.c}|{self.t}"
+
+    def build(self, num_qubits, **kwargs) -> "BuiltCNOT":
+        return BuiltCNOT(control=self.c, target=self.t, num_qubits=num_qubits)
+
+
+class BuiltCNOT(BuiltOperator):
+    """CNOT as an exact permutation (``src/qandle/operators.py:536-567``), no 4^n matrix."""
+
+    def __init__(self, control: int, target: int, num_qubits: int):
+        super().__init__()
+        self.c = control
+        self.t = target
+        self.num_qubits = num_qubits
+        self.cbit = qubit_bit(num_qubits, control)
+        self.tbit = qubit_bit(num_qubits, target)
+
+    def forward(self, state: torch.Tensor) -> torch.Tensor:
+        from ._engine.lib import require_device
+
+        require_device(state, "state")
+        st, unbatched = _as_batch(state, self.num_qubits)
+        out = plain.CnotFn.apply(st, self.num_qubits, self.cbit, self.tbit)
+        return out.squeeze(0) if unbatched else out
+
+    def __str__(self) -> str:
+        return CNOT(self.c, self.t).__str__()
+
+    def to_matrix(self, **kwargs) -> torch.Tensor:
+        N = 2 ** self.num_qubits
+        idx = torch.arange(N)
+        src = torch.where((idx & (1 << self.cbit)) != 0, idx ^ (1 << self.tbit), idx)
+        M = torch.zeros(N, N, dtype=torch.complex64)
+        M[idx, src] = 1
+        return M
+
+
+class rdict(dict):
+    """Reversible dict (``src/qandle/operators.py:701-706``)."""
+
+    @property
+    def T(self):
+        return {v: k for k, v in self.items()}
+
+
+BUILT_CLASS_RELATION = rdict(
+    {
+        UnbuiltOperator: BuiltOperator,
+        UnbuiltParametrizedOperator: BuiltParametrizedOperator,
+        RX: BuiltRX,
+        RY: BuiltRY,
+        RZ: BuiltRZ,
+        CNOT: BuiltCNOT,
+    }
+)
