@@ -97,6 +97,64 @@ int qs_pauli_im(float *out, const float *lam, const float *psi, int64_t B,
                 int n, const int32_t *bits, int d, int pauli, void *work,
                 int64_t work_bytes, void *stream);
 
+/* ---------------------------------------------------------------------------
+ * Fused pass path.  A pass loads a tile of 2^K amplitudes (K = R + TB local
+ * bit positions, lpos[] ascending with lpos[0] = 0) into LDS once, applies a
+ * run of gates in sub-passes that each hold R positions in registers, and
+ * stores it back: a whole run of gates costs one HBM round trip.  The int32
+ * `words` stream (device memory) encodes the sub-passes; its layout is in
+ * qandle_amd/csrc/qs_program.h.  Together these replace the executor loop
+ * UnsplittedCircuit.forward (src/qandle/qcircuit.py:163-174), the split
+ * executor (src/qandle/qcircuit.py:246-295) and the per-gate forwards they
+ * call (src/qandle/operators.py:289-298, 557-558).
+ * ------------------------------------------------------------------------- */
+
+/* 1 when a (register bits R, thread bits TB) kernel variant is compiled in. */
+int qs_fused_supported(int R, int TB);
+
+/* Product-state factors efac[b, pos, v] (float2) of an AngleEmbedding over bit
+ * positions bits[0..d) (host) with angles x (B, d), row stride x_stride;
+ * positions that are not embedded get (1, 0) / (0, 0) (|0>).  d = 0 gives
+ * |0..0>.  Replaces the embedding's einsum + matmul chain
+ * (src/qandle/embeddings.py:148-164): the first pass multiplies these out. */
+int qs_product_factors(float *efac, const float *x, int64_t B, int n,
+                       const int32_t *bits, int d, int rot, int64_t x_stride,
+                       void *stream);
+
+/* One pass.  src = NULL generates the product state of efac in registers;
+ * otherwise tiles are read from src (src may equal state).  cs is the float2
+ * (cos, sin) table the op words index.  mpart != NULL folds
+ * MeasureProbability partials into the pass (size from
+ * qs_fused_measure_work_bytes).  store = 0 skips writing the state back. */
+int qs_fused_pass(float *state, const float *src, int64_t B, int n, int R,
+                  int TB, const int32_t *lpos, const int32_t *words, int nsub,
+                  const float *cs, const float *efac, float *mpart, int store,
+                  void *stream);
+
+/* MeasureProbability from a measuring pass's partials: out (B, n) float32,
+ * lpos (host) = that pass's local positions.
+ * Replaces src/qandle/measurements.py:120-123. */
+int64_t qs_fused_measure_work_bytes(int64_t B, int n, int R, int TB);
+int qs_fused_measure_finish(float *out, const float *mpart, int64_t B, int n,
+                            int R, int TB, const int32_t *lpos, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Adjoint backward.  Replaces torch autograd through every dense gate of the
+ * reference (implicit; the tape of src/qandle/operators.py:272-298 and the
+ * grad_M = state^H @ g outer products it implies).  One call un-applies one
+ * forward pass: `words` is that pass re-encoded in reverse order with each
+ * rotation's gradient row; psi and lam (the co-state dL/dpsi) are updated in
+ * place, and row r of `gpart` receives Im<lam|P|psi> partials for the gate
+ * at its output.  qs_adjoint_finish turns the partials into
+ * out[r, b] = dL/dphi_r for sample b and tot[r] = sum over b.
+ * ------------------------------------------------------------------------- */
+int64_t qs_adjoint_work_bytes(int64_t B, int n, int R, int TB, int rows);
+int qs_adjoint_pass(float *psi, float *lam, int64_t B, int n, int R, int TB,
+                    const int32_t *lpos, const int32_t *words, int nsub,
+                    const float *cs, float *gpart, int rows, void *stream);
+int qs_adjoint_finish(float *out, float *tot, const float *gpart, int rows,
+                      int64_t B, int n, int R, int TB, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,162 @@
+"""Adjoint backward of a fused program, behind one torch.autograd.Function.
+
+Forward: the fused passes, keeping the final state and a copy of the state
+every ``checkpoint_gates`` gates (at pass boundaries).
+
+Backward (reverse sweep over the same passes, ``qs_adjoint_pass``):
+
+* co-state seed: lambda = 2 psi * sum_q w_q [bit n-1-q = 0] for a
+  MeasureProbability output (w = dL/dP), or dL/dpsi for a state output;
+* per pass, in reverse: for every gate, 1/2 Im<lambda|P|psi> at its output,
+  then psi <- U^dag psi, lambda <- U^dag lambda;
+* where a checkpoint exists, psi is reset to it, which bounds the float32
+  drift of un-applying gates on a deep circuit;
+* AngleEmbedding inputs: dL/dx_j = 1/2 Im<lambda_0|P_j|psi_0> (the embedded
+  factors commute, so every column comes from the one final pair);
+* an initial state that needs a gradient gets lambda_0.
+
+Gradients of remapped parameters flow through torch (``remap(theta)`` is
+computed outside the Function).
+"""
+from __future__ import annotations
+
+import torch
+
+from .. import config
+from . import fused
+from . import lib as L
+from . import plain
+
+
+def checkpoint_after(prog, pl, every_gates: int):
+    """Pass indices after which the forward state is stored."""
+    out = set()
+    done = 0
+    next_mark = every_gates
+    for i, p in enumerate(pl.passes[:-1]):
+        done += len(p.gates)
+        if every_gates > 0 and done >= next_mark:
+            out.add(i)
+            while next_mark <= done:
+                next_mark += every_gates
+    return out
+
+
+def default_checkpoint_gates(n: int) -> int:
+    """CHECKPOINT_LAYERS layers of (RX, RY, RZ per qubit + a CNOT per qubit)."""
+    return int(config.CHECKPOINT_LAYERS) * 4 * n
+
+
+class FusedCircuitFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, prog, R, TB, B, batched, names, phi_param, embed_x, state, *named_vals):
+        named = dict(zip(names, [v.detach() for v in named_vals]))
+        inp = fused.Inputs(B, batched, phi_param.device, phi_param.detach(), named,
+                           None if embed_x is None else embed_x.detach(),
+                           None if state is None else state.detach())
+        cs = fused.build_coeffs(prog, inp)
+        efac = fused.product_factors(prog, inp) if (prog.init[0] == "embed" or state is None) else None
+        pl = fused.get_plan(prog, R, TB)
+        every = config.CHECKPOINT_GATES if getattr(config, "CHECKPOINT_GATES", None) else default_checkpoint_gates(prog.n)
+        marks = checkpoint_after(prog, pl, every)
+        ckpts = {}
+
+        def keep(i, psi):
+            if i in marks:
+                ckpts[i] = psi.clone()
+
+        psi, probs = fused.execute(prog, inp, cs, efac, R, TB, want_state=True, checkpoints=keep)
+        ctx.prog, ctx.R, ctx.TB, ctx.B, ctx.names = prog, R, TB, B, names
+        ctx.ck_index = sorted(ckpts)
+        ctx.named_shapes = [v.shape for v in named_vals]
+        ctx.embed_shape = None if embed_x is None else embed_x.shape
+        ctx.state_shape = None if state is None else state.shape
+        ctx.save_for_backward(psi, cs, *[ckpts[i] for i in ctx.ck_index])
+        if prog.measure:
+            return probs
+        return psi
+
+    @staticmethod
+    def backward(ctx, gout):
+        prog, R, TB, B = ctx.prog, ctx.R, ctx.TB, ctx.B
+        saved = ctx.saved_tensors
+        psi_final, cs = saved[0], saved[1]
+        ckpts = dict(zip(ctx.ck_index, saved[2:]))
+        n = prog.n
+        lib = L.load()
+        st = L.stream_of(psi_final)
+        psi = psi_final.clone()
+        if prog.measure:
+            w = gout.to(torch.float32).contiguous()
+            lam = torch.empty_like(psi)
+            L.check(lib.qs_measure_prob_backward(L.ptr(lam), L.ptr(psi), L.ptr(w), n, B, n, st),
+                    "qs_measure_prob_backward")
+        else:
+            lam = gout.to(torch.complex64).resolve_conj().contiguous().clone()
+        pl = fused.get_plan(prog, R, TB)
+        words = fused.pass_words(prog, pl, B, psi.device, reverse=True)
+        rows = fused.rotation_rows(prog)
+        nrows = len(rows)
+        wb = lib.qs_adjoint_work_bytes(B, n, R, TB, nrows)
+        gpart = torch.empty(max(wb // 4, 1), dtype=torch.float32, device=psi.device)
+        for i in range(len(words) - 1, -1, -1):
+            w, nsub, lpos = words[i]
+            L.check(
+                lib.qs_adjoint_pass(
+                    L.ptr(psi), L.ptr(lam), B, n, R, TB,
+                    lpos.ctypes.data_as(L.ctypes.POINTER(L.ctypes.c_int32)), L.ptr(w), nsub,
+                    L.ptr(cs), L.ptr(gpart), nrows, st,
+                ),
+                "qs_adjoint_pass",
+            )
+            if (i - 1) in ckpts:
+                psi.copy_(ckpts[i - 1])
+        per = torch.empty(max(nrows, 1), B, dtype=torch.float32, device=psi.device)
+        tot = torch.empty(max(nrows, 1), dtype=torch.float32, device=psi.device)
+        L.check(lib.qs_adjoint_finish(L.ptr(per), L.ptr(tot), L.ptr(gpart), nrows, B, n, R, TB, st),
+                "qs_adjoint_finish")
+
+        # parameters (shared angles): one row each
+        g_param = None
+        if ctx.needs_input_grad[6]:
+            param_rows = []
+            for gi, g in enumerate(prog.gates):
+                if g.kind != 4 and prog.slots[g.slot].kind == "param":
+                    param_rows.append((prog.param_slots.index(g.slot), rows[gi]))
+            g_param = torch.zeros(len(prog.param_slots), dtype=torch.float32, device=psi.device)
+            if param_rows:
+                idx = torch.tensor([p for p, _ in param_rows], device=psi.device)
+                src = tot[torch.tensor([r for _, r in param_rows], device=psi.device)]
+                g_param.index_add_(0, idx, src)
+        # named inputs: per-sample rows summed over the gates that use the name
+        g_named = []
+        for k, name in enumerate(ctx.names):
+            if not ctx.needs_input_grad[9 + k]:
+                g_named.append(None)
+                continue
+            rr = [rows[gi] for gi, g in enumerate(prog.gates) if g.kind != 4 and prog.slots[g.slot].kind == "named"
+                  and prog.slots[g.slot].name == name]
+            gv = per[torch.tensor(rr, device=psi.device)].sum(0) if rr else torch.zeros(B, device=psi.device)
+            shape = ctx.named_shapes[k]
+            g_named.append(gv.sum().reshape(shape) if len(shape) == 0 else gv.reshape(shape))
+        g_embed = None
+        if ctx.needs_input_grad[7] and prog.init[0] == "embed":
+            _, rot, bits, _ = prog.init
+            gx = plain.pauli_im(lam, psi, n, list(bits), rot) * 0.5
+            shp = ctx.embed_shape
+            g_embed = gx.sum(0) if len(shp) == 1 else gx.reshape(shp)
+        g_state = None
+        if ctx.needs_input_grad[8] and prog.init[0] != "embed":
+            shp = ctx.state_shape
+            g_state = lam.sum(0) if len(shp) == 1 else (lam if lam.shape == shp else lam.sum(0, keepdim=True))
+        return (None, None, None, None, None, None, g_param, g_embed, g_state, *g_named)
+
+
+def run_with_grad(prog, inp, R, TB):
+    names = list(inp.named)
+    vals = [inp.named[k] for k in names]
+    out = FusedCircuitFn.apply(prog, R, TB, inp.B, inp.batched, names, inp.phi_param, inp.embed_x,
+                               inp.state, *vals)
+    if prog.measure:
+        return None, out
+    return out, None

@@ -1,9 +1,388 @@
-"""Circuit lowering and the fused pass engine (filled in below this stub)."""
+"""Circuit lowering and the fused-pass runner.
+
+``lower_circuit`` turns a built ``UnsplittedCircuit`` (or ``SplittedCircuit``)
+into a :class:`Program`: an initial state (|0..0>, the caller's state, or an
+AngleEmbedding product state), a list of gates on bit positions with
+coefficient slots, and whether MeasureProbability ends the circuit.  The
+program is planned once per (register bits, thread bits, cost model) and
+executed as a short sequence of ``qs_fused_pass`` launches.
+"""
+from __future__ import annotations
+
+import dataclasses
+import typing
+
+import numpy as np
+import torch
+
+from . import lib as L
+from . import planner as PL
+
+DEFAULT_TB = 8
+DEFAULT_R = 5
 
 
-def lower_circuit(circuit):
-    return None
+@dataclasses.dataclass
+class Slot:
+    """Where a rotation's angle comes from."""
+
+    kind: str                     # "param" (remap(theta) of a module) or "named"
+    module: typing.Any = None
+    name: str = ""
 
 
-def run(prog, circuit, state, kwargs):  # pragma: no cover - replaced by the fused engine
-    raise NotImplementedError
+@dataclasses.dataclass
+class Program:
+    n: int
+    gates: typing.List[PL.Gate]
+    slots: typing.List[Slot]
+    init: tuple                   # ("zero",) | ("state",) | ("embed", rot, bits, name)
+    measure: bool
+    plans: dict = dataclasses.field(default_factory=dict)
+    words_cache: dict = dataclasses.field(default_factory=dict)
+
+    @property
+    def param_slots(self):
+        return [i for i, s in enumerate(self.slots) if s.kind == "param"]
+
+    @property
+    def named_slots(self):
+        return [i for i, s in enumerate(self.slots) if s.kind == "named"]
+
+
+def _lower_into(layers, n, mapping, prog_gates, slots, state):
+    """Append the lowering of ``layers`` (qubits mapped through ``mapping``)."""
+    from .. import embeddings, measurements, operators, qcircuit
+
+    for mod in layers:
+        if isinstance(mod, operators.BuiltParametrizedOperator):
+            q = mapping(mod.qubit)
+            slot = Slot("named", name=mod.name) if mod.named else Slot("param", module=mod)
+            slots.append(slot)
+            prog_gates.append(PL.Gate(mod.KIND, n - 1 - q, -1, len(slots) - 1))
+        elif isinstance(mod, operators.BuiltCNOT):
+            prog_gates.append(PL.Gate(PL.CNOT, n - 1 - mapping(mod.t), n - 1 - mapping(mod.c)))
+        elif isinstance(mod, embeddings.AngleEmbeddingBuilt):
+            # the embedding discards the incoming state: everything before it is dead
+            prog_gates.clear()
+            slots.clear()
+            bits = [n - 1 - mapping(q) for q in mod.qubits]
+            state["init"] = ("embed", mod.kind, bits, mod.name)
+        elif isinstance(mod, measurements.MeasureProbabilityBuilt):
+            if mod.num_qubits != n or state.get("measured"):
+                return False
+            state["measured"] = True
+        elif isinstance(mod, qcircuit.Circuit):
+            if mod.num_qubits != n or state.get("measured"):
+                return False
+            inner = mod.circuit
+            if isinstance(inner, qcircuit.SplittedCircuit):
+                for sc in inner.subcircuits:
+                    inv = sc.inverse
+                    if not _lower_into(sc.layers, n, lambda s, inv=inv: mapping(inv[s]), prog_gates, slots, state):
+                        return False
+            elif not _lower_into(inner.layers, n, mapping, prog_gates, slots, state):
+                return False
+        else:
+            return False
+        if state.get("measured") and mod is not layers[-1] and not isinstance(mod, measurements.MeasureProbabilityBuilt):
+            return False
+    return True
+
+
+def lower_circuit(circuit) -> typing.Optional[Program]:
+    """Program for a built circuit, or None when a layer is outside the program format."""
+    from .. import qcircuit
+
+    cached = getattr(circuit, "_program", None)
+    if cached is not None:
+        return cached
+    n = circuit.num_qubits
+    gates: typing.List[PL.Gate] = []
+    slots: typing.List[Slot] = []
+    state = {"init": ("input",)}
+    ident = lambda q: q  # noqa: E731
+    if isinstance(circuit, qcircuit.SplittedCircuit):
+        ok = True
+        for sc in circuit.subcircuits:
+            inv = sc.inverse
+            ok = ok and _lower_into(sc.layers, n, lambda s, inv=inv: inv[s], gates, slots, state)
+    else:
+        layers = list(circuit.layers)
+        ok = _lower_into(layers, n, ident, gates, slots, state)
+        # MeasureProbability must be the last layer
+        from .. import measurements
+
+        if ok and state.get("measured"):
+            ok = isinstance(layers[-1], measurements.MeasureProbabilityBuilt) or isinstance(layers[-1], qcircuit.Circuit)
+    if not ok or n > 40 or n < 1:
+        return None
+    prog = Program(n, gates, slots, state["init"], bool(state.get("measured")))
+    circuit._program = prog
+    return prog
+
+
+def get_plan(prog: Program, R: int, TB: int, cm: PL.CostModel = None) -> PL.Plan:
+    cm = cm or DEFAULT_COST
+    key = (R, TB, None if cm is None else dataclasses.astuple(cm))
+    pl = prog.plans.get(key)
+    if pl is None:
+        pl = PL.plan(prog.n, prog.gates, R, TB, cm)
+        prog.plans[key] = pl
+    return pl
+
+
+DEFAULT_COST = PL.CostModel(budget=160.0, min_low=4)
+
+
+def choose_shape(n: int):
+    """(R, TB) for a circuit width: 32 amplitudes per thread and one wave per
+    tile (11 local bits) where the width allows, else every bit in registers
+    of a 256-thread tile that holds several samples."""
+    if n >= 5:
+        return 5, 6
+    return n, DEFAULT_TB
+
+
+# ----------------------------------------------------------------------------
+# runtime
+# ----------------------------------------------------------------------------
+
+@dataclasses.dataclass
+class Inputs:
+    B: int
+    batched: bool
+    device: torch.device
+    phi_param: torch.Tensor        # (S,) remapped angles of unnamed gates (autograd-tracked)
+    named: typing.Dict[str, torch.Tensor]
+    embed_x: typing.Optional[torch.Tensor]
+    state: typing.Optional[torch.Tensor]
+
+
+def gather_inputs(prog: Program, circuit, state, kwargs) -> Inputs:
+    device = None
+    B, batched = 1, False
+    if state is not None:
+        L.require_device(state, "state")
+        device = state.device
+        if state.dim() == 2:
+            B, batched = state.shape[0], True
+        elif state.dim() != 1:
+            raise RuntimeError("state must be (2^n,) or (B, 2^n)")
+        if state.shape[-1] != 2 ** prog.n:
+            raise RuntimeError(f"state has {state.shape[-1]} amplitudes, circuit expects {2 ** prog.n}")
+    named = {}
+    for i in prog.named_slots:
+        name = prog.slots[i].name
+        if name not in named:
+            v = kwargs[name]
+            if not isinstance(v, torch.Tensor):
+                v = torch.tensor(v, dtype=torch.float32)
+            named[name] = v
+    embed_x = None
+    if prog.init[0] == "embed":
+        embed_x = kwargs[prog.init[3]]
+        L.require_device(embed_x, f"input '{prog.init[3]}'")
+        device = embed_x.device
+        if embed_x.shape[-1] != len(prog.init[2]):
+            raise RuntimeError(f"AngleEmbedding expects {len(prog.init[2])} features, got {embed_x.shape[-1]}")
+        # the embedding discards the incoming state, so only its input sets the batch
+        if embed_x.dim() == 2:
+            B, batched = embed_x.shape[0], True
+        elif embed_x.dim() == 1:
+            B, batched = 1, False
+        else:
+            raise RuntimeError("AngleEmbedding input must be (d,) or (B, d)")
+    if device is None:
+        for v in named.values():
+            if v.device.type == "cuda":
+                device = v.device
+                break
+    if device is None:
+        for p in circuit.parameters():
+            device = p.device
+            break
+    if device is None or device.type != "cuda":
+        raise L.EngineError(
+            "qandle_amd runs on the MI355X HIP engine only; pass a state or inputs on a HIP device "
+            "or move the circuit to one (module.to('cuda'))."
+        )
+    # named (B,) inputs batch the circuit; a batch of one broadcasts against them
+    # (the reference's bmm broadcasting, src/qandle/operators.py:294-297)
+    for name in list(named):
+        v = named[name].to(device=device, dtype=torch.float32)
+        if v.dim() > 1:
+            raise RuntimeError(f"named input '{name}' must be 0-d or (B,)")
+        if v.dim() == 1:
+            if B == 1:
+                B = v.shape[0]
+            elif v.shape[0] == 1:
+                v = v.expand(B)
+            elif v.shape[0] != B:
+                raise RuntimeError(f"named input '{name}' has {v.shape[0]} entries, batch is {B}")
+            batched = True
+        named[name] = v
+    for name, v in named.items():
+        if v.dim() == 1 and v.shape[0] != B:
+            if v.shape[0] == 1:
+                named[name] = v.expand(B)
+            else:
+                raise RuntimeError(f"named input '{name}' has {v.shape[0]} entries, batch is {B}")
+    params = [prog.slots[i].module for i in prog.param_slots]
+    if params:
+        phis = [m.remapping(m.theta).reshape(-1)[:1].to(device=device, dtype=torch.float32) for m in params]
+        phi_param = torch.cat(phis)
+    else:
+        phi_param = torch.zeros(0, dtype=torch.float32, device=device)
+    return Inputs(B, batched, device, phi_param, named, embed_x, state)
+
+
+def coefficient_layout(prog: Program, B: int):
+    """float2 index of every slot in the coefficient table, and its per-sample flag."""
+    index = {}
+    per_sample = {}
+    off = 0
+    for i in prog.param_slots:
+        index[i] = off
+        per_sample[i] = False
+        off += 1
+    name_off = {}
+    for i in prog.named_slots:
+        nm = prog.slots[i].name
+        if nm not in name_off:
+            name_off[nm] = off
+            off += B
+        index[i] = name_off[nm]
+        per_sample[i] = True
+    return index, per_sample, off, name_off
+
+
+def rotation_rows(prog: Program):
+    """Gradient row of every rotation gate (program order)."""
+    rows = {}
+    for gi, g in enumerate(prog.gates):
+        if g.kind != PL.CNOT:
+            rows[gi] = len(rows)
+    return rows
+
+
+def pass_words(prog: Program, pl: PL.Plan, B: int, device, reverse: bool = False) -> list:
+    key = (id(pl), B, str(device), reverse)
+    cached = prog.words_cache.get(key)
+    if cached is not None:
+        return cached
+    index, per_sample, _, _ = coefficient_layout(prog, B)
+    rows = rotation_rows(prog) if reverse else None
+    out = []
+    for p in pl.passes:
+        w = PL.encode_pass(p, prog.gates, pl.R, pl.TB, lambda s: index[s], lambda s: per_sample[s],
+                           reverse=reverse, row_of=(rows.__getitem__ if reverse else None))
+        out.append((torch.from_numpy(w).to(device), len(p.subs), np.asarray(p.L, dtype=np.int32)))
+    prog.words_cache[key] = out
+    return out
+
+
+def build_coeffs(prog: Program, inp: Inputs) -> torch.Tensor:
+    """(entries, 2) float32 (cos, sin) of every slot's half-angle, on the device."""
+    from .plain import half_angle_coeffs
+
+    parts = [inp.phi_param.detach()]
+    _, _, total, name_off = coefficient_layout(prog, inp.B)
+    for nm in name_off:
+        v = inp.named[nm].detach()
+        parts.append(v.expand(inp.B) if v.dim() == 0 else v)
+    phi = torch.cat([p.reshape(-1) for p in parts]) if parts else torch.zeros(0, device=inp.device)
+    if phi.numel() == 0:
+        phi = torch.zeros(1, device=inp.device)
+    return half_angle_coeffs(phi)
+
+
+def product_factors(prog: Program, inp: Inputs) -> torch.Tensor:
+    n = prog.n
+    efac = torch.empty(inp.B, n, 2, 2, dtype=torch.float32, device=inp.device)
+    lib = L.load()
+    if prog.init[0] == "embed":
+        _, rot, bits, _ = prog.init
+        x = inp.embed_x.detach().to(torch.float32)
+        if x.dim() == 1:
+            x = x.unsqueeze(0)
+        if x.shape[0] != inp.B:
+            x = x.expand(inp.B, -1)
+        x = x.contiguous()
+        L.check(
+            lib.qs_product_factors(L.ptr(efac), L.ptr(x), inp.B, n, L.i32_array(bits), len(bits), rot,
+                                   x.shape[1], L.stream_of(efac)),
+            "qs_product_factors",
+        )
+    else:
+        L.check(lib.qs_product_factors(L.ptr(efac), None, inp.B, n, None, 0, 0, 0, L.stream_of(efac)),
+                "qs_product_factors")
+    return efac
+
+
+def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
+            cm: PL.CostModel = None, want_state: bool = True, checkpoints=None):
+    """Runs every pass.  Returns (state or None, probs or None)."""
+    n, B = prog.n, inp.B
+    pl = get_plan(prog, R, TB, cm)
+    words = pass_words(prog, pl, B, inp.device)
+    lib = L.load()
+    st = L.stream_of(cs)
+    psi = torch.empty(B, 2 ** n, dtype=torch.complex64, device=inp.device)
+    src = None
+    if prog.init[0] == "input" and inp.state is not None:
+        s = inp.state.to(torch.complex64).resolve_conj()
+        if s.dim() == 1:
+            s = s.unsqueeze(0)
+        if s.shape[0] != B:
+            s = s.expand(B, -1)
+        src = s.contiguous()
+    mpart = probs = None
+    if prog.measure:
+        wb = lib.qs_fused_measure_work_bytes(B, n, R, TB)
+        mpart = torch.empty(max(wb // 4, 1), dtype=torch.float32, device=inp.device)
+    npass = len(words)
+    for i, (w, nsub, lpos) in enumerate(words):
+        last = i == npass - 1
+        store = 1 if (not last or want_state or not prog.measure) else 0
+        L.check(
+            lib.qs_fused_pass(
+                L.ptr(psi), None if src is None else L.ptr(src), B, n, R, TB,
+                lpos.ctypes.data_as(L.ctypes.POINTER(L.ctypes.c_int32)), L.ptr(w), nsub,
+                L.ptr(cs), None if efac is None else L.ptr(efac),
+                L.ptr(mpart) if (last and mpart is not None) else None, store, st,
+            ),
+            "qs_fused_pass",
+        )
+        src = psi
+        if checkpoints is not None:
+            checkpoints(i, psi)
+    if prog.measure:
+        probs = torch.empty(B, n, dtype=torch.float32, device=inp.device)
+        lpos = words[-1][2]
+        L.check(
+            lib.qs_fused_measure_finish(L.ptr(probs), L.ptr(mpart), B, n, R, TB,
+                                        lpos.ctypes.data_as(L.ctypes.POINTER(L.ctypes.c_int32)), st),
+            "qs_fused_measure_finish",
+        )
+    return psi, probs
+
+
+def run(prog: Program, circuit, state, kwargs):
+    """Forward of a lowered circuit (with the adjoint backward when gradients are needed)."""
+    from .. import measurements
+
+    inp = gather_inputs(prog, circuit, state, kwargs)
+    from . import adjoint
+
+    out_state, probs = adjoint.forward(prog, inp)
+    if prog.measure:
+        P = probs
+        return measurements.reference_shape(P) if not inp.batched else _batched_probs(P)
+    return out_state if inp.batched else out_state.squeeze(0)
+
+
+def _batched_probs(P: torch.Tensor) -> torch.Tensor:
+    from .. import measurements
+
+    return measurements.reference_shape(P)

@@ -38,6 +38,14 @@ SIGNATURES = {
     "qs_measure_prob_backward": (I32, [P, P, P, I64, I64, I32, P]),
     "qs_pauli_im_work_bytes": (I64, [I64, I32, I32]),
     "qs_pauli_im": (I32, [P, P, P, I64, I32, P, I32, I32, P, I64, P]),
+    "qs_fused_supported": (I32, [I32, I32]),
+    "qs_product_factors": (I32, [P, P, I64, I32, P, I32, I32, I64, P]),
+    "qs_fused_pass": (I32, [P, P, I64, I32, I32, I32, P, P, I32, P, P, P, I32, P]),
+    "qs_fused_measure_work_bytes": (I64, [I64, I32, I32, I32]),
+    "qs_fused_measure_finish": (I32, [P, P, I64, I32, I32, I32, P, P]),
+    "qs_adjoint_work_bytes": (I64, [I64, I32, I32, I32, I32]),
+    "qs_adjoint_pass": (I32, [P, P, I64, I32, I32, I32, P, P, I32, P, P, I32, P]),
+    "qs_adjoint_finish": (I32, [P, P, P, I32, I64, I32, I32, I32, P]),
 }
 
 

@@ -8,6 +8,6 @@ from . import qw_map
 DEFAULT_MAPPING = qw_map.tanh
 
 # Engine knobs (no counterpart in the reference).
-# Gate runs wider than this many register/LDS bits are split across passes.
 FUSED = True            # Circuit.forward uses the fused pass engine
 CHECKPOINT_LAYERS = 5   # adjoint backward restores psi from a checkpoint every k layers
+CHECKPOINT_GATES = None  # explicit checkpoint spacing in gates (overrides CHECKPOINT_LAYERS)

@@ -22,6 +22,10 @@ FLAGS = [
     "--offload-arch=gfx950",
     # no packed-FP32 (v_pk_*_f32) code generation
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # keep SimplifyCFG from sinking the per-case register updates of the op
+    # dispatch switches into a common block: that turns the register-array
+    # index into a phi and forces the whole array into scratch memory
+    "-mllvm", "-simplifycfg-sink-common=false",
     "-Wno-unused-command-line-argument",
 ]
 

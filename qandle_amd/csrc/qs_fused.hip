@@ -1,0 +1,442 @@
+// Fused pass kernel for gfx950: many gates per HBM round trip.
+//
+// One workgroup owns one tile: the 2^K amplitudes whose non-local index bits
+// are fixed (K = R + TB local bits; the low local bits are the lowest global
+// positions so loads/stores are contiguous 16-byte vectors).  The tile is
+// staged in LDS.  A pass is a list of sub-passes; in each sub-pass every
+// thread holds the 2^R amplitudes that differ in the sub-pass's R "register"
+// bits and applies that sub-pass's gates entirely in VGPRs (register indices
+// are compile-time, the dispatch over bit positions is a uniform branch).
+// Gates whose second bit is not a register bit are still possible when that
+// bit is only a CNOT control or an RZ: the bit's value is then constant per
+// thread.  The first pass of a circuit generates the AngleEmbedding product
+// state in registers instead of loading; the last pass can fold
+// MeasureProbability into a deterministic reduction.
+//
+// Replaces the executor loop of UnsplittedCircuit.forward
+// (src/qandle/qcircuit.py:163-174) and the dense per-gate `state @ M` it runs
+// (src/qandle/operators.py:289-298, 557-558), the embedding's matrix chain
+// (src/qandle/embeddings.py:148-164) and MeasureProbability's masked reduce
+// (src/qandle/measurements.py:120-123).
+#include "qs_pass_common.h"
+
+namespace qs {
+
+struct PassArgs {
+  float *state;
+  const float *src;
+  const float *cs;
+  const float *efac;
+  float *mpart;
+  const int32_t *words;
+  int64_t B;
+  int n;
+  int nsub;
+  int store;
+  int multi;  // tile spans 2^(K-n) whole samples (n < K)
+  int nn;     // number of non-local qubit positions (n > K)
+  int32_t lpos[QS_MAX_LOCAL_BITS];
+  int32_t npos[QS_MAX_QUBITS];
+};
+
+template <int R, int TB, bool MULTI>
+__global__ void __launch_bounds__(1 << TB) k_pass(PassArgs A) {
+  constexpr int T = 1 << TB;
+  constexpr int K = R + TB;
+  constexpr int NR = 1 << R;
+  __shared__ __align__(16) float2 tile[1 << K];
+  __shared__ float red[(T / 64) * (K + 1)];
+
+  const int tid = threadIdx.x;
+  const int n = A.n;
+  const int64_t total = A.B << n;
+  const int64_t t = blockIdx.x;
+  int64_t base;
+  if (MULTI) {
+    base = t << K;
+  } else {
+    const int64_t tl = t & ((int64_t(1) << A.nn) - 1);
+    base = (t >> A.nn) << n;
+    for (int i = 0; i < A.nn; ++i)
+      if ((tl >> i) & 1) base |= int64_t(1) << A.npos[i];
+  }
+  const int64_t block_sample = base >> n;
+
+  // global offsets of this thread's load/store slots
+  int64_t g_tid = 0;
+#pragma unroll
+  for (int i = 0; i < TB; ++i)
+    if ((tid >> i) & 1) g_tid |= int64_t(1) << A.lpos[i + 1];
+
+  if (A.src) {
+#pragma unroll
+    for (int it = 0; it < (NR >> 1); ++it) {
+      int64_t g_it = 0;
+#pragma unroll
+      for (int i = 0; i < R - 1; ++i)
+        if ((it >> i) & 1) g_it |= int64_t(1) << A.lpos[TB + 1 + i];
+      const int64_t gi = base | g_tid | g_it;
+      float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (gi < total) val = reinterpret_cast<const float4 *>(A.src)[gi >> 1];
+      const int l = 2 * (it * T + tid);
+      *reinterpret_cast<float4 *>(&tile[swz(l)]) = val;
+    }
+    __syncthreads();
+  }
+
+  float2 v[NR];
+  const int32_t *w = A.words;
+  for (int s = 0; s < A.nsub; ++s) {
+    const int nops = w[0];
+    int gl = 0;
+    int64_t gbase = base;
+#pragma unroll
+    for (int i = 0; i < TB; ++i) {
+      if ((tid >> i) & 1) {
+        gl |= 1 << w[kHdrG + i];
+        gbase |= int64_t(1) << w[kHdrGG + i];
+      }
+    }
+    const int64_t sample = MULTI ? (gbase >> n) : block_sample;
+    const bool live = sample < A.B;
+    const int lb = swz(gl);
+    int sw[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) sw[r] = w[kHdrSwz + r];
+    // LDS index of register j: swz is linear over XOR, so it is lb ^ (xor of the
+    // swizzled strides of j's set bits)
+    auto addr = [&](int j) {
+      int a = lb;
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if ((j >> r) & 1) a ^= sw[r];
+      return a;
+    };
+
+    if (s == 0 && !A.src) {
+      // product-state generator: (x)_pos f_pos(bit), per sample
+      if (live) {
+        const float2 *E = reinterpret_cast<const float2 *>(A.efac) + sample * n * 2;
+        int64_t rmask = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) rmask |= int64_t(1) << w[kHdrGP + r];
+        float2 pre = make_float2(1.f, 0.f);
+        for (int pos = 0; pos < n; ++pos)
+          if (!((rmask >> pos) & 1)) pre = cmul(pre, E[pos * 2 + ((gbase >> pos) & 1)]);
+        // doubling over the register bits; both loops have constant trip
+        // counts so every v[] index is a compile-time constant
+        v[0] = pre;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const float2 f0 = E[w[kHdrGP + r] * 2], f1 = E[w[kHdrGP + r] * 2 + 1];
+#pragma unroll
+          for (int j = 0; j < (1 << r); ++j) {
+            v[j | (1 << r)] = cmul(v[j], f1);
+            v[j] = cmul(v[j], f0);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < NR; ++j) v[j] = make_float2(0.f, 0.f);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < NR; ++j) v[j] = tile[addr(j)];
+    }
+
+    // a padding sample past B (last tile of a multi-sample pass) holds zeros;
+    // it reads sample 0's coefficients so every table access stays in bounds
+    run_ops<R>(v, w + kHdr, nops, reinterpret_cast<const float2 *>(A.cs), live ? sample : 0, gbase);
+
+    if (s == A.nsub - 1 && A.mpart) {
+      // MeasureProbability partials: for every local bit, sum of |a|^2 over the
+      // tile's amplitudes (of this sample) with that bit = 0; then the total.
+      float p[NR];
+      float tot = 0.f;
+#pragma unroll
+      for (int j = 0; j < NR; ++j) {
+        p[j] = v[j].x * v[j].x + v[j].y * v[j].y;
+        tot += p[j];
+      }
+      float c[K + 1];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        float acc = 0.f;
+#pragma unroll
+        for (int j = 0; j < NR; ++j)
+          if (!(j & (1 << r))) acc += p[j];
+        c[r] = acc;
+      }
+#pragma unroll
+      for (int i = 0; i < TB; ++i) c[R + i] = ((tid >> i) & 1) ? 0.f : tot;
+      c[K] = tot;
+      // reduce over the threads of one sample (a power-of-two tid segment)
+      const int seglog = MULTI ? (n - R) : TB;
+      const int wlog = seglog < 6 ? seglog : 6;
+#pragma unroll
+      for (int e = 0; e <= K; ++e) {
+        float x = c[e];
+        for (int off = 1; off < (1 << wlog); off <<= 1) x += __shfl_xor(x, off, 64);
+        c[e] = x;
+      }
+      const int spt_log = MULTI ? (K - n) : 0;
+      float *mp = A.mpart + (t << spt_log) * (K + 1);
+      if (seglog <= 6) {
+        if ((tid & ((1 << seglog) - 1)) == 0) {
+          float *o = mp + (tid >> seglog) * (K + 1);
+#pragma unroll
+          for (int r = 0; r < R; ++r) o[w[kHdrP + r]] = c[r];
+#pragma unroll
+          for (int i = 0; i < TB; ++i) o[w[kHdrG + i]] = c[R + i];
+          o[K] = c[K];
+        }
+      } else {
+        const int wave = tid >> 6;
+        if ((tid & 63) == 0) {
+#pragma unroll
+          for (int e = 0; e <= K; ++e) red[wave * (K + 1) + e] = c[e];
+        }
+        __syncthreads();
+        const int wps = 1 << (seglog - 6);  // waves per sample
+        const int nsamp = (T / 64) / wps;
+        if (tid < nsamp * (K + 1)) {
+          const int sm = tid / (K + 1), e = tid % (K + 1);
+          float x = 0.f;
+          for (int q = 0; q < wps; ++q) x += red[(sm * wps + q) * (K + 1) + e];
+          int dst;
+          if (e < R)
+            dst = w[kHdrP + e];
+          else if (e < K)
+            dst = w[kHdrG + e - R];
+          else
+            dst = K;
+          mp[sm * (K + 1) + dst] = x;
+        }
+      }
+    }
+
+#pragma unroll
+    for (int j = 0; j < NR; ++j) tile[addr(j)] = v[j];
+    __syncthreads();
+    w += kHdr + 4 * nops;
+  }
+
+  if (A.store) {
+#pragma unroll
+    for (int it = 0; it < (NR >> 1); ++it) {
+      int64_t g_it = 0;
+#pragma unroll
+      for (int i = 0; i < R - 1; ++i)
+        if ((it >> i) & 1) g_it |= int64_t(1) << A.lpos[TB + 1 + i];
+      const int64_t gi = base | g_tid | g_it;
+      const int l = 2 * (it * T + tid);
+      if (gi < total)
+        reinterpret_cast<float4 *>(A.state)[gi >> 1] = *reinterpret_cast<const float4 *>(&tile[swz(l)]);
+    }
+  }
+}
+
+// P[b, q] from the per-tile partials of the pass that measured.
+struct FinishArgs {
+  int32_t loc_of_pos[QS_MAX_QUBITS];  // local bit index of a global position, -1 if not local
+  int32_t nl_of_pos[QS_MAX_QUBITS];   // index into the tile number's bits, -1 if local
+};
+
+__global__ void __launch_bounds__(256)
+k_measure_finish(float *out, const float *mpart, int n, int K, int multi, int nn, FinishArgs F) {
+  __shared__ double red[4];
+  const int64_t b = blockIdx.x;
+  const int q = blockIdx.y;
+  const int pos = n - 1 - q;
+  double acc = 0.0;
+  if (multi) {
+    if (threadIdx.x == 0) {
+      const int64_t t = b >> (K - n), s = b & ((int64_t(1) << (K - n)) - 1);
+      acc = mpart[((t << (K - n)) + s) * (K + 1) + pos];
+    }
+  } else {
+    const int64_t ntl = int64_t(1) << nn;
+    const int lp = F.loc_of_pos[pos], np = F.nl_of_pos[pos];
+    for (int64_t tl = threadIdx.x; tl < ntl; tl += 256) {
+      const float *pp = mpart + (b * ntl + tl) * (K + 1);
+      if (lp >= 0)
+        acc += pp[lp];
+      else if (!((tl >> np) & 1))
+        acc += pp[K];
+    }
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[b * n + q] = (float)(red[0] + red[1] + red[2] + red[3]);
+}
+
+// (B, n, 2) product-state factors of an AngleEmbedding (or |0> for unembedded bits)
+struct FactorArgs {
+  int32_t slot_of_bit[QS_MAX_QUBITS];
+  int rot;
+};
+
+__global__ void k_product_factors(float *efac, const float *x, int64_t B, int n, int64_t x_stride,
+                                  FactorArgs a) {
+  const int64_t count = B * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = e / n;
+    const int pos = (int)(e % n);
+    const int j = a.slot_of_bit[pos];
+    float2 f0 = make_float2(1.f, 0.f), f1 = make_float2(0.f, 0.f);
+    if (j >= 0 && x) {
+      float s, c;
+      sincosf(x[b * x_stride + j] * 0.5f, &s, &c);
+      if (a.rot == QS_RX) {
+        f0 = make_float2(c, 0.f);
+        f1 = make_float2(0.f, -s);
+      } else if (a.rot == QS_RY) {
+        f0 = make_float2(c, 0.f);
+        f1 = make_float2(s, 0.f);
+      } else {
+        f0 = make_float2(c, -s);
+      }
+    }
+    float2 *o = reinterpret_cast<float2 *>(efac) + e * 2;
+    o[0] = f0;
+    o[1] = f1;
+  }
+}
+
+template <int R, int TB>
+static int launch_pass(const PassArgs &A, int64_t ntiles, hipStream_t st) {
+  if (A.multi)
+    hipLaunchKernelGGL((k_pass<R, TB, true>), dim3((unsigned)ntiles), dim3(1 << TB), 0, st, A);
+  else
+    hipLaunchKernelGGL((k_pass<R, TB, false>), dim3((unsigned)ntiles), dim3(1 << TB), 0, st, A);
+  return launch_status();
+}
+
+}  // namespace qs
+
+using namespace qs;
+
+static int64_t fused_tiles(int64_t B, int n, int K) {
+  if (n >= K) return B << (n - K);
+  const int64_t spt = int64_t(1) << (K - n);
+  return (B + spt - 1) / spt;
+}
+
+extern "C" {
+
+int qs_fused_supported(int R, int TB) {
+  return (TB == 8 && R >= 1 && R <= 5) || (TB == 7 && R == 5) || (TB == 6 && R == 5);
+}
+
+int64_t qs_fused_measure_work_bytes(int64_t B, int n, int R, int TB) {
+  if (B < 1 || n < 1 || n > QS_MAX_QUBITS || !qs_fused_supported(R, TB)) return -1;
+  const int K = R + TB;
+  const int64_t spt = n >= K ? 1 : (int64_t(1) << (K - n));
+  return fused_tiles(B, n, K) * spt * (K + 1) * (int64_t)sizeof(float);
+}
+
+int qs_fused_pass(float *state, const float *src, int64_t B, int n, int R, int TB,
+                  const int32_t *lpos, const int32_t *words, int nsub, const float *cs,
+                  const float *efac, float *mpart, int store, void *stream) {
+  if (B < 1 || n < 1 || n > QS_MAX_QUBITS || !state || !lpos || !words || nsub < 1 ||
+      !qs_fused_supported(R, TB) || R > n || (!src && !efac))
+    return QS_EINVAL;
+  const int K = R + TB;
+  PassArgs A;
+  A.state = state;
+  A.src = src;
+  A.cs = cs;
+  A.efac = efac;
+  A.mpart = mpart;
+  A.words = words;
+  A.B = B;
+  A.n = n;
+  A.nsub = nsub;
+  A.store = store;
+  A.multi = n < K;
+  for (int j = 0; j < QS_MAX_LOCAL_BITS; ++j) A.lpos[j] = 0;
+  for (int j = 0; j < QS_MAX_QUBITS; ++j) A.npos[j] = 0;
+  if (A.multi) {
+    for (int j = 0; j < K; ++j) A.lpos[j] = j;
+    A.nn = 0;
+  } else {
+    int64_t seen = 0;
+    for (int j = 0; j < K; ++j) {
+      if (lpos[j] < 0 || lpos[j] >= n || ((seen >> lpos[j]) & 1)) return QS_EINVAL;
+      if (j > 0 && lpos[j] <= lpos[j - 1]) return QS_EINVAL;
+      seen |= int64_t(1) << lpos[j];
+      A.lpos[j] = lpos[j];
+    }
+    if (A.lpos[0] != 0) return QS_EINVAL;  // amplitude pairs must be contiguous
+    int c = 0;
+    for (int p = 0; p < n; ++p)
+      if (!((seen >> p) & 1)) A.npos[c++] = p;
+    A.nn = c;
+  }
+  const int64_t ntiles = fused_tiles(B, n, K);
+  if (ntiles > 0x7fffffff) return QS_EUNSUPPORTED;
+  hipStream_t st = (hipStream_t)stream;
+  if (TB == 8) {
+    switch (R) {
+      case 1: return launch_pass<1, 8>(A, ntiles, st);
+      case 2: return launch_pass<2, 8>(A, ntiles, st);
+      case 3: return launch_pass<3, 8>(A, ntiles, st);
+      case 4: return launch_pass<4, 8>(A, ntiles, st);
+      case 5: return launch_pass<5, 8>(A, ntiles, st);
+    }
+  }
+  if (TB == 7 && R == 5) return launch_pass<5, 7>(A, ntiles, st);
+  if (TB == 6 && R == 5) return launch_pass<5, 6>(A, ntiles, st);
+  return QS_EUNSUPPORTED;
+}
+
+int qs_fused_measure_finish(float *out, const float *mpart, int64_t B, int n, int R, int TB,
+                            const int32_t *lpos, void *stream) {
+  if (B < 1 || n < 1 || n > QS_MAX_QUBITS || !out || !mpart || !lpos ||
+      !qs_fused_supported(R, TB))
+    return QS_EINVAL;
+  const int K = R + TB;
+  const int multi = n < K;
+  FinishArgs F;
+  for (int p = 0; p < QS_MAX_QUBITS; ++p) F.loc_of_pos[p] = F.nl_of_pos[p] = -1;
+  int nn = 0;
+  if (multi) {
+    for (int p = 0; p < n; ++p) F.loc_of_pos[p] = p;
+  } else {
+    for (int j = 0; j < K; ++j) {
+      if (lpos[j] < 0 || lpos[j] >= n) return QS_EINVAL;
+      F.loc_of_pos[lpos[j]] = j;
+    }
+    for (int p = 0; p < n; ++p)
+      if (F.loc_of_pos[p] < 0) F.nl_of_pos[p] = nn++;
+  }
+  if (B > 0x7fffffff) return QS_EUNSUPPORTED;
+  hipLaunchKernelGGL(k_measure_finish, dim3((unsigned)B, (unsigned)n), dim3(256), 0,
+                     (hipStream_t)stream, out, mpart, n, K, multi, nn, F);
+  return launch_status();
+}
+
+int qs_product_factors(float *efac, const float *x, int64_t B, int n, const int32_t *bits, int d,
+                       int rot, int64_t x_stride, void *stream) {
+  if (B < 1 || n < 1 || n > QS_MAX_QUBITS || !efac || d < 0 || d > n || rot < 0 || rot > 2 ||
+      (d > 0 && (!x || !bits)))
+    return QS_EINVAL;
+  FactorArgs a;
+  for (int p = 0; p < QS_MAX_QUBITS; ++p) a.slot_of_bit[p] = -1;
+  for (int j = 0; j < d; ++j) {
+    if (bits[j] < 0 || bits[j] >= n || a.slot_of_bit[bits[j]] >= 0) return QS_EINVAL;
+    a.slot_of_bit[bits[j]] = j;
+  }
+  a.rot = rot;
+  const int64_t count = B * n;
+  int64_t blocks = (count + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_product_factors, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     efac, d > 0 ? x : nullptr, B, n, x_stride, a);
+  return launch_status();
+}
+
+}  // extern "C"

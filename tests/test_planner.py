@@ -1,0 +1,130 @@
+"""Fusion planner + word encoder, checked on the CPU through the kernel emulator.
+
+The emulator executes the exact int32 words the GPU kernel receives; the
+result must equal the FP64 oracle for the same circuit.  Covers tiles that hold
+several samples (n < K), tiles inside one sample (n >= K, several tiles per
+sample), CNOT controls outside the registers, RZ on non-register bits, and
+dependency order across passes.
+"""
+import numpy as np
+import pytest
+
+from oracle import fp64
+from qandle_amd._engine import planner as PL
+from tests.fused_emulator import emulate_program
+
+KIND = {"rx": PL.RX, "ry": PL.RY, "rz": PL.RZ}
+
+
+def lower(n, ops, B):
+    """oracle ops -> (gates, cs table, efac)."""
+    gates, cs = [], []
+    efac = np.zeros((B, n, 2), dtype=np.complex128)
+    efac[:, :, 0] = 1.0
+    for op in ops:
+        if op[0] == "embed":
+            _, rot, qubits, X = op
+            X = np.broadcast_to(np.asarray(X, dtype=np.float64), (B, len(qubits)))
+            for j, q in enumerate(qubits):
+                U = fp64.rot_matrix(rot, X[:, j])
+                efac[:, n - 1 - q, 0] = U[:, 0, 0]
+                efac[:, n - 1 - q, 1] = U[:, 1, 0]
+        elif op[0] == "cnot":
+            gates.append(PL.Gate(PL.CNOT, n - 1 - op[2], n - 1 - op[1]))
+        else:
+            gates.append(PL.Gate(KIND[op[0]], n - 1 - op[1], -1, len(cs)))
+            t = op[2] / 2
+            cs.append((np.cos(t), np.sin(t)))
+    cs = np.asarray(cs if cs else [(1.0, 0.0)], dtype=np.float64)
+    return gates, cs, efac
+
+
+def random_ops(rng, n, depth, B, cnot_mode="ladder"):
+    ops = [("embed", "x", list(range(n)), rng.uniform(0, np.pi, size=(B, n)))]
+    for _ in range(depth):
+        for q in range(n):
+            for k in ("rx", "ry", "rz"):
+                if rng.random() < 0.7:
+                    ops.append((k, q, float(rng.uniform(0, 2 * np.pi))))
+        if cnot_mode == "ladder":
+            for q in range(n - 1):
+                ops.append(("cnot", q, q + 1))
+        else:
+            for _ in range(n):
+                c, t = rng.choice(n, 2, replace=False)
+                ops.append(("cnot", int(c), int(t)))
+    return ops
+
+
+@pytest.mark.parametrize(
+    "n,B,R,TB,mode",
+    [
+        (6, 5, 3, 4, "ladder"),    # n < K: multi-sample tiles, partial last tile
+        (7, 3, 5, 2, "ladder"),    # n = K
+        (10, 2, 3, 4, "ladder"),   # 8 tiles per sample
+        (10, 2, 3, 4, "random"),   # long-range CNOTs
+        (11, 2, 2, 3, "random"),   # tiny registers: many sub-passes and passes
+        (12, 1, 5, 4, "ladder"),
+    ],
+)
+def test_emulated_plan_matches_oracle(n, B, R, TB, mode):
+    rng = np.random.default_rng(n * 31 + R)
+    ops = random_ops(rng, n, 3, B, mode)
+    gates, cs, efac = lower(n, ops, B)
+    cm = PL.CostModel(budget=40.0, min_low=2)
+    state, probs, pl = emulate_program(gates, n, B, R, TB, cs, efac, cm=cm)
+    ref = fp64.simulate(n, ops)
+    np.testing.assert_allclose(state, ref, atol=1e-12)
+    np.testing.assert_allclose(probs, fp64.measure_prob(ref, n), atol=1e-12)
+    if n >= R + TB:
+        assert len(pl.passes) >= 2
+
+
+def test_emulated_input_state_and_no_gates():
+    rng = np.random.default_rng(4)
+    n, B = 9, 2
+    psi0 = rng.normal(size=(B, 2 ** n)) + 1j * rng.normal(size=(B, 2 ** n))
+    gates, cs, efac = lower(n, [], B)
+    state, probs, pl = emulate_program(gates, n, B, 3, 4, cs, efac, src=psi0)
+    np.testing.assert_allclose(state, psi0, atol=1e-15)
+    np.testing.assert_allclose(probs, fp64.measure_prob(psi0, n), atol=1e-12)
+
+
+def test_plan_respects_dependencies():
+    """Every gate runs exactly once and no gate runs before an earlier one on a shared position."""
+    rng = np.random.default_rng(8)
+    n = 16
+    ops = random_ops(rng, n, 4, 1, "random")
+    gates, _, _ = lower(n, ops, 1)
+    pl = PL.plan(n, gates, 5, 8)
+    order = [g for p in pl.passes for sp in p.subs for g in sp.gates]
+    assert sorted(order) == list(range(len(gates)))
+    when = {g: i for i, g in enumerate(order)}
+    for i, gi in enumerate(gates):
+        for j in range(i):
+            if set(gates[j].positions) & set(gi.positions):
+                assert when[j] < when[i]
+    for p in pl.passes:
+        assert len(p.L) == 13 and p.L == sorted(p.L) and p.L[0] == 0
+        for sp in p.subs:
+            assert set(sp.P) <= set(p.L)
+            for g in sp.gates:
+                assert set(gates[g].need) <= set(sp.P)
+
+
+def test_pass_count_c2_family():
+    """20 qubits, 10 layers of RX,RY,RZ + ladder: far fewer passes than gates."""
+    n = 20
+    gates = []
+    slot = 0
+    for _ in range(10):
+        for q in range(n):
+            for k in (PL.RX, PL.RY, PL.RZ):
+                gates.append(PL.Gate(k, n - 1 - q, -1, slot))
+                slot += 1
+        for q in range(n - 1):
+            gates.append(PL.Gate(PL.CNOT, n - 2 - q, n - 1 - q))
+    pl = PL.plan(n, gates, 5, 8)
+    stats = PL.plan_stats(pl, gates)
+    print(f"C2 family: {len(gates)} gates in {len(pl.passes)} passes;", [s["gates"] for s in stats])
+    assert len(pl.passes) <= 60
