@@ -1,0 +1,261 @@
+"""Benchmark of the qandle gate-application hot path on one MI355X.
+
+Headline (the JSON line's ``value``): amplitude-gate updates per second of the
+fused forward on the plain-forward workload: 20 qubits, batch 64, complex64,
+RX AngleEmbedding (inputs U[0, pi)), 10 layers of RX, RY, RZ on every qubit
+(angles U[0, 2 pi)) + CNOT ladder q -> q+1, MeasureProbability.  One step =
+one forward of that circuit over the batch, inputs already resident in HBM.
+Updates per step = sum over gate applications of B * 2^n (each embedded qubit
+counts as one gate).
+
+Also reported: the plain per-gate GPU path on the same workload, the fused
+forward at 24 qubits (batch 4), the adjoint forward + backward at 16 qubits
+(batch 256, 20 layers) against the plain per-gate path, the roofline of the
+dominant kernel (the fused pass kernel), and the CPU restatement of qandle's
+complex64 torch path timed on a bounded sample on this host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+CPU_THREADS = 16
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--quick", action="store_true", help="headline only (no extra workloads, no CPU baseline)")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def fused_bytes_per_launch(B, n, npass, stores_last):
+    """Algorithmic HBM bytes of the executed schedule, averaged per pass launch.
+
+    First pass: generates the embedding product state in registers and writes
+    it (8 B/amplitude).  Middle passes: read + write (16 B).  Last pass: read
+    (8 B), plus a write when the state is kept.  Measurement partials are
+    (K+1) floats per tile, negligible.
+    """
+    amps = B * 2 ** n
+    per_amp = 8 + 16 * max(npass - 2, 0) + (8 + (8 if stores_last else 0) if npass > 1 else 0)
+    if npass == 1:
+        per_amp = 8 if stores_last else 0
+    return amps * per_amp / npass, amps * per_amp
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build targets a single MI355X "
+              "(multi-GPU sharding is designed, not built; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import qandle_amd as qandle
+    from qandle_amd import workloads
+    from qandle_amd._engine import fused, lib
+
+    lib.load()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+
+    def sync():
+        torch.cuda.synchronize(dev)
+
+    def time_steps(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        sync()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        sync()
+        wall = time.perf_counter() - t0
+        return e0.elapsed_time(e1) / 1e3, wall
+
+    # ---------------------------------------------------------------- headline
+    n, B, depth = 20, 64, 10
+    w = workloads.rxryrz_ladder(n=n, B=B, depth=depth, seed=1)
+    circ = qandle.Circuit(w.layers, num_qubits=n).to(dev)
+    x = w.inputs["x"].to(dev)
+    prog = fused.lower_circuit(circ.circuit)
+    R, TB = fused.choose_shape(n)
+
+    def step():
+        with torch.no_grad():
+            return circ(x=x)
+
+    gpu_s, wall_s = time_steps(step, args.steps, args.warmup)
+    ms_per_step = gpu_s / args.steps * 1e3
+    updates = w.amplitude_updates()
+    value = updates * args.steps / gpu_s
+
+    # per-launch durations of the dominant kernel, on its own stream, over K steps
+    plan = fused.get_plan(prog, R, TB)
+    npass = len(plan.passes)
+    with torch.no_grad():
+        inp = fused.gather_inputs(prog, circ.circuit, None, {"x": x})
+        cs = fused.build_coeffs(prog, inp)
+        efac = fused.product_factors(prog, inp)
+        events = []
+        for _ in range(max(2, args.steps // 4)):
+            fused.execute(prog, inp, cs, efac, R, TB, want_state=False, events=events)
+        sync()
+    durs = [a.elapsed_time(b) / 1e3 for a, b in events]
+    avg_launch = sum(durs) / len(durs)
+    bpl, bstep = fused_bytes_per_launch(B, n, npass, stores_last=False)
+    achieved = bpl / avg_launch / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_fused_pass_n20_b64.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    result = {
+        "metric": "amplitude-gate updates/s (fused forward); % HBM peak; fused vs plain; p50/p99 serve latency",
+        "value": value,
+        "unit": "amplitude-gate updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic (seeded inputs and angles, remapping=None)",
+        "config": {
+            "workload": "plain forward: 20 qubits, batch 64, RX AngleEmbedding + 10 x (RX,RY,RZ per qubit + CNOT ladder) + MeasureProbability",
+            "qubits": n, "batch": B, "layers": depth, "gate_ops": w.gate_ops,
+            "updates_per_step": updates, "fused_passes": npass, "tile": {"R": R, "TB": TB},
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"k_pass<{R},{TB},false> (qs_fused_pass)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "avg_launch_us": avg_launch * 1e6,
+            "alg_bytes_per_launch": bpl,
+            "launches_timed": len(durs),
+        },
+        "wall_s_timed_region": wall_s,
+    }
+
+    if not args.quick:
+        extras = {}
+        # plain per-gate GPU path on the same workload
+        def plain_step():
+            with torch.no_grad():
+                return circ.circuit.forward_layers(x=x)
+        p_s, _ = time_steps(plain_step, 2, 1)
+        extras["plain_updates_per_s"] = updates * 2 / p_s
+        extras["plain_ms_per_step"] = p_s / 2 * 1e3
+        extras["fused_over_plain"] = extras["plain_ms_per_step"] / ms_per_step
+        extras["fused_alg_bytes_per_step"] = bstep
+        extras["fused_step_hbm_frac"] = bstep / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS
+        extras["plain_step_hbm_frac"] = (B * 2 ** n * (8 + 16 * len(w.gates) + 8)) / (p_s / 2) / 1e9 / HBM_PEAK_GBS
+
+        # 24 qubits, batch 4 (same circuit family)
+        w24 = workloads.rxryrz_ladder(n=24, B=4, depth=10, seed=1)
+        c24 = qandle.Circuit(w24.layers, num_qubits=24).to(dev)
+        x24 = w24.inputs["x"].to(dev)
+        with torch.no_grad():
+            t24, _ = time_steps(lambda: c24(x=x24), 5, 2)
+        p24 = fused.lower_circuit(c24.circuit)
+        np24 = len(fused.get_plan(p24, *fused.choose_shape(24)).passes)
+        _, b24 = fused_bytes_per_launch(4, 24, np24, stores_last=False)
+        extras["n24_updates_per_s"] = w24.amplitude_updates() * 5 / t24
+        extras["n24_ms_per_step"] = t24 / 5 * 1e3
+        extras["n24_hbm_frac"] = b24 / (t24 / 5) / 1e9 / HBM_PEAK_GBS
+        extras["n24_passes"] = np24
+
+        # adjoint forward + backward, 16 qubits, batch 256, 20 layers
+        w3 = workloads.rxryrz_ladder(n=16, B=256, depth=20, seed=2, loss=True)
+        c3 = qandle.Circuit(w3.layers, num_qubits=16).to(dev)
+        x3 = w3.inputs["x"].to(dev).requires_grad_(True)
+        lw = w3.loss_w.to(dev)
+
+        def adj_step():
+            c3.zero_grad(set_to_none=True)
+            x3.grad = None
+            (c3(x=x3) * lw).sum().backward()
+
+        def plain_adj_step():
+            c3.zero_grad(set_to_none=True)
+            x3.grad = None
+            (c3.circuit.forward_layers(x=x3) * lw).sum().backward()
+
+        ta, _ = time_steps(adj_step, 3, 1)
+        tp, _ = time_steps(plain_adj_step, 1, 1)
+        extras["adjoint_fwd_bwd_ms"] = ta / 3 * 1e3
+        extras["plain_autograd_fwd_bwd_ms"] = tp * 1e3
+        extras["adjoint_speedup_vs_plain"] = tp / (ta / 3)
+        result["extras"] = extras
+
+        if not args.no_cpu:
+            result["cpu_baseline"] = cpu_baseline(w)
+    print(json.dumps(result))
+
+
+def cpu_baseline(w):
+    """qandle's complex64 torch semantics restated (oracle/qandle_c64.py, structured
+    mode: the dense reference needs 2 x 8 TiB of cached matrices per gate at 20
+    qubits) on a bounded sample: the full circuit for 2 of the 64 samples."""
+    import torch
+
+    from oracle import qandle_c64
+
+    threads = min(CPU_THREADS, os.cpu_count() or 1)
+    torch.set_num_threads(threads)
+    Bs = 2
+    ops = w.torch_ops()
+    ops[0] = (ops[0][0], ops[0][1], ops[0][2], ops[0][3][:Bs])
+    qandle_c64.structured_run(w.n, ops[:4], Bs)  # warm-up
+    t0 = time.perf_counter()
+    qandle_c64.structured_run(w.n, ops, Bs)
+    dt = time.perf_counter() - t0
+    updates = w.gate_ops * Bs * 2 ** w.n
+    return {
+        "value": updates / dt,
+        "unit": "amplitude-gate updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"full {w.gate_ops}-gate circuit, {Bs} of {w.B} samples, torch CPU complex64 (structured restatement), {dt:.1f} s",
+        "cpu": _cpu_model(),
+    }
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+if __name__ == "__main__":
+    main()

@@ -321,8 +321,11 @@ def product_factors(prog: Program, inp: Inputs) -> torch.Tensor:
 
 
 def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
-            cm: PL.CostModel = None, want_state: bool = True, checkpoints=None):
-    """Runs every pass.  Returns (state or None, probs or None)."""
+            cm: PL.CostModel = None, want_state: bool = True, checkpoints=None, events=None):
+    """Runs every pass.  Returns (state or None, probs or None).
+
+    ``events``: optional list; a (start, end) pair of HIP events recorded on the
+    launch stream around every pass kernel is appended to it."""
     n, B = prog.n, inp.B
     pl = get_plan(prog, R, TB, cm)
     words = pass_words(prog, pl, B, inp.device)
@@ -345,6 +348,9 @@ def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
     for i, (w, nsub, lpos) in enumerate(words):
         last = i == npass - 1
         store = 1 if (not last or want_state or not prog.measure) else 0
+        if events is not None:
+            ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            ev[0].record(torch.cuda.current_stream(inp.device))
         L.check(
             lib.qs_fused_pass(
                 L.ptr(psi), None if src is None else L.ptr(src), B, n, R, TB,
@@ -354,6 +360,9 @@ def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
             ),
             "qs_fused_pass",
         )
+        if events is not None:
+            ev[1].record(torch.cuda.current_stream(inp.device))
+            events.append(ev)
         src = psi
         if checkpoints is not None:
             checkpoints(i, psi)
