@@ -145,15 +145,66 @@ int qs_fused_measure_finish(float *out, const float *mpart, int64_t B, int n,
  * forward pass: `words` is that pass re-encoded in reverse order with each
  * rotation's gradient row; psi and lam (the co-state dL/dpsi) are updated in
  * place, and row r of `gpart` receives Im<lam|P|psi> partials for the gate
- * at its output.  qs_adjoint_finish turns the partials into
+ * at its output.  seed_w != NULL (the first call of a sweep) builds the
+ * MeasureProbability co-state from psi while loading instead of reading lam:
+ * lam = 2 psi * sum_q seed_w[b, q] [bit n-1-q = 0], seed_w (B, n) float32.
+ * qs_adjoint_finish turns the partials into
  * out[r, b] = dL/dphi_r for sample b and tot[r] = sum over b.
  * ------------------------------------------------------------------------- */
 int64_t qs_adjoint_work_bytes(int64_t B, int n, int R, int TB, int rows);
 int qs_adjoint_pass(float *psi, float *lam, int64_t B, int n, int R, int TB,
                     const int32_t *lpos, const int32_t *words, int nsub,
-                    const float *cs, float *gpart, int rows, void *stream);
+                    const float *cs, float *gpart, int rows,
+                    const float *seed_w, void *stream);
 int qs_adjoint_finish(float *out, float *tot, const float *gpart, int rows,
                       int64_t B, int n, int R, int TB, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Work-item form for batches of uneven circuits.  A descriptor is an opaque
+ * record (size from the *_desc_bytes query) filled on the host from exactly
+ * the arguments of the direct call; the caller copies an array of them to
+ * device memory.  `items` (device) holds int32 pairs (descriptor index, tile
+ * index); one launch runs every item with a grid-stride loop over `grid`
+ * workgroups, so circuits of different widths, depths and programs share one
+ * launch and can be captured into a single hipGraph.  Replaces running the
+ * reference's per-circuit Python executor (src/qandle/qcircuit.py:163-174)
+ * once per circuit.
+ * ------------------------------------------------------------------------- */
+/* Product-state factors of many circuits in one launch: entry e writes
+ * efac[e] = (f(0), f(1)) (two float2) from x[entries[2e]] with rotation
+ * entries[2e+1], or |0> when entries[2e] is -1.  The per-circuit equivalent
+ * is qs_product_factors. */
+int qs_factor_entries(float *efac, const float *x, const int32_t *entries,
+                      int64_t count, void *stream);
+
+int64_t qs_pass_desc_bytes(void);
+int qs_pass_desc(void *desc, int64_t *ntiles, float *state, const float *src,
+                 int64_t B, int n, int R, int TB, const int32_t *lpos,
+                 const int32_t *words, int nsub, const float *cs,
+                 const float *efac, float *mpart, int store);
+int qs_items_pass(const void *descs, const int32_t *items, int nitems, int R,
+                  int TB, int grid, void *stream);
+
+int64_t qs_measure_desc_bytes(void);
+int qs_measure_desc(void *desc, float *out, const float *mpart, int64_t B,
+                    int n, int R, int TB, const int32_t *lpos);
+int qs_items_measure_finish(const void *descs, int ndesc, int max_n,
+                            void *stream);
+
+int64_t qs_adjoint_desc_bytes(void);
+int qs_adjoint_desc(void *desc, int64_t *ntiles, float *psi, float *lam,
+                    int64_t B, int n, int R, int TB, const int32_t *lpos,
+                    const int32_t *words, int nsub, const float *cs,
+                    float *gpart, int rows, const float *seed_w);
+int qs_items_adjoint(const void *descs, const int32_t *items, int nitems,
+                     int R, int TB, int grid, void *stream);
+
+int64_t qs_adjoint_finish_desc_bytes(void);
+int qs_adjoint_finish_desc(void *desc, float *out, float *tot,
+                           const float *gpart, int rows, int64_t B, int n,
+                           int R, int TB);
+int qs_items_adjoint_finish(const void *descs, int ndesc, int row_blocks,
+                            void *stream);
 
 #ifdef __cplusplus
 }

@@ -25,7 +25,6 @@ import torch
 from .. import config
 from . import fused
 from . import lib as L
-from . import plain
 
 
 def checkpoint_after(prog, pl, every_gates: int):
@@ -86,34 +85,40 @@ class FusedCircuitFn(torch.autograd.Function):
         lib = L.load()
         st = L.stream_of(psi_final)
         psi = psi_final.clone()
+        lam = torch.empty_like(psi)
+        seed = None
         if prog.measure:
-            w = gout.to(torch.float32).contiguous()
-            lam = torch.empty_like(psi)
-            L.check(lib.qs_measure_prob_backward(L.ptr(lam), L.ptr(psi), L.ptr(w), n, B, n, st),
-                    "qs_measure_prob_backward")
+            # the first adjoint pass builds lambda from psi and w = dL/dP while loading
+            seed = gout.to(torch.float32).contiguous()
         else:
-            lam = gout.to(torch.complex64).resolve_conj().contiguous().clone()
+            lam.copy_(gout.to(torch.complex64).resolve_conj())
         pl = fused.get_plan(prog, R, TB)
         words = fused.pass_words(prog, pl, B, psi.device, reverse=True)
         rows = fused.rotation_rows(prog)
         nrows = len(rows)
-        wb = lib.qs_adjoint_work_bytes(B, n, R, TB, nrows)
+        embed_rows = 0
+        gwords = []
+        if prog.init[0] == "embed" and ctx.needs_input_grad[7]:
+            gplan, gwords = embed_grad_words(prog, R, TB, B, psi.device, nrows)
+            embed_rows = len(prog.init[2])
+        total_rows = nrows + embed_rows
+        wb = lib.qs_adjoint_work_bytes(B, n, R, TB, total_rows)
         gpart = torch.empty(max(wb // 4, 1), dtype=torch.float32, device=psi.device)
-        for i in range(len(words) - 1, -1, -1):
-            w, nsub, lpos = words[i]
+        sweep = [(words[i], i) for i in range(len(words) - 1, -1, -1)] + [(gw, None) for gw in gwords]
+        for k, ((w, nsub, lpos), i) in enumerate(sweep):
             L.check(
                 lib.qs_adjoint_pass(
                     L.ptr(psi), L.ptr(lam), B, n, R, TB,
                     lpos.ctypes.data_as(L.ctypes.POINTER(L.ctypes.c_int32)), L.ptr(w), nsub,
-                    L.ptr(cs), L.ptr(gpart), nrows, st,
+                    L.ptr(cs), L.ptr(gpart), total_rows, None if (k > 0 or seed is None) else L.ptr(seed), st,
                 ),
                 "qs_adjoint_pass",
             )
-            if (i - 1) in ckpts:
+            if i is not None and (i - 1) in ckpts:
                 psi.copy_(ckpts[i - 1])
-        per = torch.empty(max(nrows, 1), B, dtype=torch.float32, device=psi.device)
-        tot = torch.empty(max(nrows, 1), dtype=torch.float32, device=psi.device)
-        L.check(lib.qs_adjoint_finish(L.ptr(per), L.ptr(tot), L.ptr(gpart), nrows, B, n, R, TB, st),
+        per = torch.empty(max(total_rows, 1), B, dtype=torch.float32, device=psi.device)
+        tot = torch.empty(max(total_rows, 1), dtype=torch.float32, device=psi.device)
+        L.check(lib.qs_adjoint_finish(L.ptr(per), L.ptr(tot), L.ptr(gpart), total_rows, B, n, R, TB, st),
                 "qs_adjoint_finish")
 
         # parameters (shared angles): one row each
@@ -140,9 +145,8 @@ class FusedCircuitFn(torch.autograd.Function):
             shape = ctx.named_shapes[k]
             g_named.append(gv.sum().reshape(shape) if len(shape) == 0 else gv.reshape(shape))
         g_embed = None
-        if ctx.needs_input_grad[7] and prog.init[0] == "embed":
-            _, rot, bits, _ = prog.init
-            gx = plain.pauli_im(lam, psi, n, list(bits), rot) * 0.5
+        if embed_rows:
+            gx = per[nrows:total_rows].t()  # (B, d), already 1/2 Im<lam|P_j|psi>
             shp = ctx.embed_shape
             g_embed = gx.sum(0) if len(shp) == 1 else gx.reshape(shp)
         g_state = None
@@ -150,6 +154,28 @@ class FusedCircuitFn(torch.autograd.Function):
             shp = ctx.state_shape
             g_state = lam.sum(0) if len(shp) == 1 else (lam if lam.shape == shp else lam.sum(0, keepdim=True))
         return (None, None, None, None, None, None, g_param, g_embed, g_state, *g_named)
+
+
+def embed_grad_words(prog, R, TB, B, device, row0):
+    """Adjoint passes that only reduce 1/2 Im<lam|P_j|psi> for every embedded bit
+    (rows row0 + j); the factors of the product state commute, so one pair
+    (psi_0, lam_0) gives every column."""
+    key = ("embed_grad", R, TB, B, str(device), row0)
+    hit = prog.words_cache.get(key)
+    if hit is not None:
+        return hit
+    from . import planner as PL
+
+    _, rot, bits, _ = prog.init
+    gates = [PL.Gate(PL.PGRAD + rot, b, -1, j) for j, b in enumerate(bits)]
+    gplan = PL.plan(prog.n, gates, R, TB, fused.DEFAULT_COST)
+    out = []
+    for p in gplan.passes:
+        w = PL.encode_pass(p, gates, R, TB, lambda s_: 0, lambda s_: False, reverse=True,
+                           row_of=lambda gi: row0 + gi)
+        out.append((torch.from_numpy(w).to(device), len(p.subs), __import__("numpy").asarray(p.L, dtype="int32")))
+    prog.words_cache[key] = (gplan, out)
+    return gplan, out
 
 
 def run_with_grad(prog, inp, R, TB):

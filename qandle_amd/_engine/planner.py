@@ -31,8 +31,11 @@ import numpy as np
 
 # op kinds (match qs_program.h for rotations)
 RX, RY, RZ, CNOT = 0, 1, 2, 4
-OP_RX, OP_RY, OP_RZ, OP_RZ_UNIFORM, OP_CNOT, OP_CNOT_UNIFORM = 0, 1, 2, 3, 4, 5
+# gradient-only Pauli reductions (adjoint kernel): kind = PGRAD + Pauli (0 X, 1 Y, 2 Z)
+PGRAD = 8
+OP_RX, OP_RY, OP_RZ, OP_RZ_UNIFORM, OP_CNOT, OP_CNOT_UNIFORM, OP_PAULI_GRAD = 0, 1, 2, 3, 4, 5, 6
 OP_PER_SAMPLE = 0x100
+PAULI_UNIFORM = 0x10
 
 HDR = 56
 HDR_P, HDR_SWZ, HDR_GP, HDR_G, HDR_GG = 1, 8, 16, 24, 40
@@ -52,7 +55,7 @@ class Gate:
     @property
     def need(self):
         """Positions that must be register bits for the gate to run."""
-        if self.kind == RZ:
+        if self.kind == RZ or self.kind == PGRAD + 2:
             return ()
         return (self.pos,)
 
@@ -238,6 +241,13 @@ def encode_pass(p: Pass, gates, R: int, TB: int, coef_index, per_sample,
         reg = {q: r for r, q in enumerate(sp.P)}
         for gi in (reversed(sp.gates) if reverse else sp.gates):
             g = gates[gi]
+            if g.kind >= PGRAD:
+                pauli = g.kind - PGRAD
+                if g.pos in reg:
+                    words.extend([OP_PAULI_GRAD, reg[g.pos], row_of(gi), pauli])
+                else:
+                    words.extend([OP_PAULI_GRAD, g.pos, row_of(gi), pauli | PAULI_UNIFORM])
+                continue
             if g.kind == CNOT:
                 if g.ctrl in reg:
                     words.extend([OP_CNOT, reg[g.pos], reg[g.ctrl], 0])

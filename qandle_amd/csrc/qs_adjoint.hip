@@ -14,6 +14,8 @@
 // Reductions go per (gate, tile segment) to a workspace in a fixed order;
 // qs_adjoint_finish sums them per sample in double.  No float atomics, so
 // gradients are bit-reproducible.
+#include <string.h>
+
 #include "qs_pass_common.h"
 
 namespace qs {
@@ -24,6 +26,7 @@ struct AdjArgs {
   const float *cs;
   float *gpart;
   const int32_t *words;
+  const float *seed_w;  // non-null: lam = 2 psi sum_q w[b, q] [bit n-1-q = 0], (B, n) row-major
   int64_t B;
   int64_t NS;  // reduction slots per gradient row
   int n;
@@ -56,17 +59,14 @@ __device__ __forceinline__ float im_dispatch(const float2 (&l)[1 << R], const fl
 }
 
 template <int R, int TB, bool MULTI>
-__global__ void __launch_bounds__(1 << TB) k_adjoint_pass(AdjArgs A) {
+__device__ __forceinline__ void adjoint_tile(const AdjArgs &A, const int64_t t, float2 *tp, float2 *tl) {
   constexpr int T = 1 << TB;
   constexpr int K = R + TB;
   constexpr int NR = 1 << R;
-  __shared__ __align__(16) float2 tp[1 << K];
-  __shared__ __align__(16) float2 tl[1 << K];
 
   const int tid = threadIdx.x;
   const int n = A.n;
   const int64_t total = A.B << n;
-  const int64_t t = blockIdx.x;
   int64_t base;
   if (MULTI) {
     base = t << K;
@@ -97,7 +97,18 @@ __global__ void __launch_bounds__(1 << TB) k_adjoint_pass(AdjArgs A) {
     float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
     if (gi < total) {
       a = reinterpret_cast<const float4 *>(A.psi)[gi >> 1];
-      b = reinterpret_cast<const float4 *>(A.lam)[gi >> 1];
+      if (A.seed_w) {
+        // co-state seed of MeasureProbability for the pair (gi, gi + 1)
+        const float *wb = A.seed_w + (gi >> n) * n;
+        float w0 = 0.f, w1 = 0.f;
+        for (int pos = 1; pos < n; ++pos)
+          if (!((gi >> pos) & 1)) w0 += wb[n - 1 - pos];
+        w1 = w0;
+        w0 += wb[n - 1];  // position 0 is 0 for gi (even) and 1 for gi + 1
+        b = make_float4(2.f * w0 * a.x, 2.f * w0 * a.y, 2.f * w1 * a.z, 2.f * w1 * a.w);
+      } else {
+        b = reinterpret_cast<const float4 *>(A.lam)[gi >> 1];
+      }
     }
     const int l = swz(2 * (it * T + tid));
     *reinterpret_cast<float4 *>(&tp[l]) = a;
@@ -155,9 +166,27 @@ __global__ void __launch_bounds__(1 << TB) k_adjoint_pass(AdjArgs A) {
         }
         continue;
       }
+      float g;
+      if (kind == QS_OP_PAULI_GRAD) {
+        const int pauli = op[3] & 3;
+        if (op[3] & QS_PAULI_UNIFORM) {  // Z on a bit that is constant per thread
+          float acc = 0.f;
+#pragma unroll
+          for (int j = 0; j < NR; ++j) acc += vl[j].x * vp[j].y - vl[j].y * vp[j].x;
+          g = ((gbase >> a) & 1) ? -acc : acc;
+        } else if (pauli == QS_RX) {
+          g = im_dispatch<QS_RX, R>(vl, vp, a);
+        } else if (pauli == QS_RY) {
+          g = im_dispatch<QS_RY, R>(vl, vp, a);
+        } else {
+          g = im_dispatch<QS_RZ, R>(vl, vp, a);
+        }
+        for (int off = 1; off < (1 << wlog); off <<= 1) g += __shfl_xor(g, off, 64);
+        if (leader) A.gpart[(int64_t)b * A.NS + u] = g;
+        continue;
+      }
       const int64_t idx = op[3] + ((code & QS_OP_PER_SAMPLE) ? csample : 0);
       const float2 c_s = cs[idx];
-      float g;
       if (kind == QS_OP_RX) {
         g = im_dispatch<QS_RX, R>(vl, vp, a);
         rot_dispatch<QS_RX, R>(vp, a, c_s.x, -c_s.y);
@@ -214,24 +243,67 @@ __global__ void __launch_bounds__(1 << TB) k_adjoint_pass(AdjArgs A) {
   }
 }
 
+template <int R, int TB, bool MULTI>
+__global__ void __launch_bounds__(1 << TB) k_adjoint_pass(AdjArgs A) {
+  __shared__ __align__(16) float2 tp[1 << (R + TB)];
+  __shared__ __align__(16) float2 tl[1 << (R + TB)];
+  adjoint_tile<R, TB, MULTI>(A, blockIdx.x, tp, tl);
+}
+
+template <int R, int TB>
+__global__ void __launch_bounds__(1 << TB)
+k_adjoint_items(const AdjArgs *descs, const int32_t *items, int nitems) {
+  __shared__ __align__(16) float2 tp[1 << (R + TB)];
+  __shared__ __align__(16) float2 tl[1 << (R + TB)];
+  for (int i = blockIdx.x; i < nitems; i += gridDim.x) {
+    const AdjArgs &A = descs[items[2 * i]];
+    const int64_t t = items[2 * i + 1];
+    if (A.n < R + TB)
+      adjoint_tile<R, TB, true>(A, t, tp, tl);
+    else
+      adjoint_tile<R, TB, false>(A, t, tp, tl);
+    __syncthreads();
+  }
+}
+
 // out[row, b] = 1/2 sum of the row's slots of sample b; tot[row] = sum over b.
-__global__ void __launch_bounds__(256)
-k_adjoint_finish(float *out, float *tot, const float *gpart, int64_t B, int64_t NS, int64_t sps) {
-  __shared__ double red[4];
-  const int64_t row = blockIdx.x;
-  const float *g = gpart + row * NS;
+struct AdjFinishArgs {
+  float *out;
+  float *tot;
+  const float *gpart;
+  int64_t B;
+  int64_t NS;
+  int64_t sps;
+  int rows;
+};
+
+__device__ __forceinline__ void adjoint_finish_row(const AdjFinishArgs &F, int64_t row, double *red) {
+  const float *g = F.gpart + row * F.NS;
   double all = 0.0;
-  for (int64_t b = threadIdx.x; b < B; b += 256) {
+  for (int64_t b = threadIdx.x; b < F.B; b += 256) {
     double acc = 0.0;
-    for (int64_t k = 0; k < sps; ++k) acc += g[b * sps + k];
-    out[row * B + b] = (float)(0.5 * acc);
+    for (int64_t k = 0; k < F.sps; ++k) acc += g[b * F.sps + k];
+    F.out[row * F.B + b] = (float)(0.5 * acc);
     all += 0.5 * acc;
   }
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) all += __shfl_xor(all, off, 64);
+  __syncthreads();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = all;
   __syncthreads();
-  if (threadIdx.x == 0) tot[row] = (float)(red[0] + red[1] + red[2] + red[3]);
+  if (threadIdx.x == 0) F.tot[row] = (float)(red[0] + red[1] + red[2] + red[3]);
+}
+
+__global__ void __launch_bounds__(256) k_adjoint_finish(AdjFinishArgs F) {
+  __shared__ double red[4];
+  adjoint_finish_row(F, blockIdx.x, red);
+}
+
+// one block per (circuit, row block)
+__global__ void __launch_bounds__(256) k_adjoint_finish_items(const AdjFinishArgs *descs) {
+  __shared__ double red[4];
+  const AdjFinishArgs &F = descs[blockIdx.x];
+  for (int64_t row = blockIdx.y; row < F.rows; row += gridDim.y) adjoint_finish_row(F, row, red);
 }
 
 }  // namespace qs
@@ -267,29 +339,30 @@ int64_t qs_adjoint_work_bytes(int64_t B, int n, int R, int TB, int rows) {
   return (int64_t)(rows > 0 ? rows : 1) * NS * (int64_t)sizeof(float);
 }
 
-int qs_adjoint_pass(float *psi, float *lam, int64_t B, int n, int R, int TB, const int32_t *lpos,
-                    const int32_t *words, int nsub, const float *cs, float *gpart, int rows,
-                    void *stream) {
+}  // extern "C"
+
+static int fill_adj_args(AdjArgs &A, int64_t *ntiles, float *psi, float *lam, int64_t B, int n, int R,
+                         int TB, const int32_t *lpos, const int32_t *words, int nsub, const float *cs,
+                         float *gpart, int rows, const float *seed_w) {
   if (B < 1 || n < 1 || n > QS_MAX_QUBITS || !psi || !lam || !lpos || !words || nsub < 1 ||
       !gpart || rows < 0 || !qs_fused_supported(R, TB) || R > n)
     return QS_EINVAL;
   const int K = R + TB;
-  AdjArgs A;
   A.psi = psi;
   A.lam = lam;
   A.cs = cs;
   A.gpart = gpart;
   A.words = words;
+  A.seed_w = seed_w;
   A.B = B;
   A.n = n;
   A.nsub = nsub;
-  int64_t ntiles, NS, sps;
-  adj_geometry(B, n, R, TB, &ntiles, &NS, &sps);
+  int64_t NS, sps;
+  adj_geometry(B, n, R, TB, ntiles, &NS, &sps);
   A.NS = NS;
   for (int j = 0; j < QS_MAX_LOCAL_BITS; ++j) A.lpos[j] = 0;
   for (int j = 0; j < QS_MAX_QUBITS; ++j) A.npos[j] = 0;
-  const bool multi = n < K;
-  if (multi) {
+  if (n < K) {
     for (int j = 0; j < K; ++j) A.lpos[j] = j;
     A.nn = 0;
   } else {
@@ -306,7 +379,20 @@ int qs_adjoint_pass(float *psi, float *lam, int64_t B, int n, int R, int TB, con
       if (!((seen >> p) & 1)) A.npos[c++] = p;
     A.nn = c;
   }
+  return QS_OK;
+}
+
+extern "C" {
+
+int qs_adjoint_pass(float *psi, float *lam, int64_t B, int n, int R, int TB, const int32_t *lpos,
+                    const int32_t *words, int nsub, const float *cs, float *gpart, int rows,
+                    const float *seed_w, void *stream) {
+  AdjArgs A;
+  int64_t ntiles;
+  int st0 = fill_adj_args(A, &ntiles, psi, lam, B, n, R, TB, lpos, words, nsub, cs, gpart, rows, seed_w);
+  if (st0) return st0;
   if (ntiles > 0x7fffffff) return QS_EUNSUPPORTED;
+  const bool multi = n < R + TB;
   hipStream_t st = (hipStream_t)stream;
   if (TB == 8) {
     switch (R) {
@@ -318,20 +404,88 @@ int qs_adjoint_pass(float *psi, float *lam, int64_t B, int n, int R, int TB, con
     }
   }
   if (TB == 7 && R == 5) return launch_adj<5, 7>(A, multi, ntiles, st);
+  if (TB == 7 && R == 4) return launch_adj<4, 7>(A, multi, ntiles, st);
   if (TB == 6 && R == 5) return launch_adj<5, 6>(A, multi, ntiles, st);
   return QS_EUNSUPPORTED;
 }
 
-int qs_adjoint_finish(float *out, float *tot, const float *gpart, int rows, int64_t B, int n,
-                      int R, int TB, void *stream) {
+int64_t qs_adjoint_desc_bytes(void) { return (int64_t)sizeof(AdjArgs); }
+
+int qs_adjoint_desc(void *desc, int64_t *ntiles, float *psi, float *lam, int64_t B, int n, int R,
+                    int TB, const int32_t *lpos, const int32_t *words, int nsub, const float *cs,
+                    float *gpart, int rows, const float *seed_w) {
+  if (!desc || !ntiles) return QS_EINVAL;
+  AdjArgs A;
+  int st0 = fill_adj_args(A, ntiles, psi, lam, B, n, R, TB, lpos, words, nsub, cs, gpart, rows, seed_w);
+  if (st0) return st0;
+  memcpy(desc, &A, sizeof(AdjArgs));
+  return QS_OK;
+}
+
+int qs_items_adjoint(const void *descs, const int32_t *items, int nitems, int R, int TB, int grid,
+                     void *stream) {
+  if (!descs || !items || nitems < 0 || grid < 1) return QS_EINVAL;
+  if (nitems == 0) return QS_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const AdjArgs *d = (const AdjArgs *)descs;
+  const unsigned g = (unsigned)(grid < nitems ? grid : nitems);
+  if (R == 4 && TB == 7) {
+    hipLaunchKernelGGL((k_adjoint_items<4, 7>), dim3(g), dim3(128), 0, st, d, items, nitems);
+    return launch_status();
+  }
+  if (R == 4 && TB == 8) {
+    hipLaunchKernelGGL((k_adjoint_items<4, 8>), dim3(g), dim3(256), 0, st, d, items, nitems);
+    return launch_status();
+  }
+  if (R == 5 && TB == 6) {
+    hipLaunchKernelGGL((k_adjoint_items<5, 6>), dim3(g), dim3(64), 0, st, d, items, nitems);
+    return launch_status();
+  }
+  return QS_EUNSUPPORTED;
+}
+
+static int fill_adj_finish(AdjFinishArgs &F, float *out, float *tot, const float *gpart, int rows,
+                           int64_t B, int n, int R, int TB) {
   if (B < 1 || n < 1 || n > QS_MAX_QUBITS || rows < 0 || !qs_fused_supported(R, TB) || R > n)
     return QS_EINVAL;
+  if (rows > 0 && (!out || !tot || !gpart)) return QS_EINVAL;
+  int64_t ntiles;
+  F.out = out;
+  F.tot = tot;
+  F.gpart = gpart;
+  F.B = B;
+  F.rows = rows;
+  adj_geometry(B, n, R, TB, &ntiles, &F.NS, &F.sps);
+  return QS_OK;
+}
+
+int qs_adjoint_finish(float *out, float *tot, const float *gpart, int rows, int64_t B, int n,
+                      int R, int TB, void *stream) {
+  AdjFinishArgs F;
+  int st0 = fill_adj_finish(F, out, tot, gpart, rows, B, n, R, TB);
+  if (st0) return st0;
   if (rows == 0) return QS_OK;
-  if (!out || !tot || !gpart) return QS_EINVAL;
-  int64_t ntiles, NS, sps;
-  adj_geometry(B, n, R, TB, &ntiles, &NS, &sps);
-  hipLaunchKernelGGL(k_adjoint_finish, dim3((unsigned)rows), dim3(256), 0, (hipStream_t)stream, out,
-                     tot, gpart, B, NS, sps);
+  hipLaunchKernelGGL(k_adjoint_finish, dim3((unsigned)rows), dim3(256), 0, (hipStream_t)stream, F);
+  return launch_status();
+}
+
+int64_t qs_adjoint_finish_desc_bytes(void) { return (int64_t)sizeof(AdjFinishArgs); }
+
+int qs_adjoint_finish_desc(void *desc, float *out, float *tot, const float *gpart, int rows,
+                           int64_t B, int n, int R, int TB) {
+  if (!desc) return QS_EINVAL;
+  AdjFinishArgs F;
+  int st0 = fill_adj_finish(F, out, tot, gpart, rows, B, n, R, TB);
+  if (st0) return st0;
+  memcpy(desc, &F, sizeof(AdjFinishArgs));
+  return QS_OK;
+}
+
+int qs_items_adjoint_finish(const void *descs, int ndesc, int row_blocks, void *stream) {
+  if (!descs || ndesc < 0 || row_blocks < 1) return QS_EINVAL;
+  if (ndesc == 0) return QS_OK;
+  hipLaunchKernelGGL(k_adjoint_finish_items, dim3((unsigned)ndesc, (unsigned)row_blocks), dim3(256),
+                     0, (hipStream_t)stream, (const AdjFinishArgs *)descs);
   return launch_status();
 }
 

@@ -18,6 +18,8 @@
 // (src/qandle/operators.py:289-298, 557-558), the embedding's matrix chain
 // (src/qandle/embeddings.py:148-164) and MeasureProbability's masked reduce
 // (src/qandle/measurements.py:120-123).
+#include <string.h>
+
 #include "qs_pass_common.h"
 
 namespace qs {
@@ -39,18 +41,17 @@ struct PassArgs {
   int32_t npos[QS_MAX_QUBITS];
 };
 
+// One tile of one pass.  `tile` (2^K float2, 16-byte aligned) and `red`
+// ((T/64)*(K+1) floats) are the caller's LDS.
 template <int R, int TB, bool MULTI>
-__global__ void __launch_bounds__(1 << TB) k_pass(PassArgs A) {
+__device__ __forceinline__ void pass_tile(const PassArgs &A, const int64_t t, float2 *tile, float *red) {
   constexpr int T = 1 << TB;
   constexpr int K = R + TB;
   constexpr int NR = 1 << R;
-  __shared__ __align__(16) float2 tile[1 << K];
-  __shared__ float red[(T / 64) * (K + 1)];
 
   const int tid = threadIdx.x;
   const int n = A.n;
   const int64_t total = A.B << n;
-  const int64_t t = blockIdx.x;
   int64_t base;
   if (MULTI) {
     base = t << K;
@@ -236,29 +237,58 @@ __global__ void __launch_bounds__(1 << TB) k_pass(PassArgs A) {
   }
 }
 
+template <int R, int TB, bool MULTI>
+__global__ void __launch_bounds__(1 << TB) k_pass(PassArgs A) {
+  __shared__ __align__(16) float2 tile[1 << (R + TB)];
+  __shared__ float red[((1 << TB) / 64) * (R + TB + 1)];
+  pass_tile<R, TB, MULTI>(A, blockIdx.x, tile, red);
+}
+
+// Work-item form: item i runs tile items[2i+1] of the pass described by
+// descs[items[2i]].  Circuits of different widths and programs share a launch.
+template <int R, int TB>
+__global__ void __launch_bounds__(1 << TB)
+k_pass_items(const PassArgs *descs, const int32_t *items, int nitems) {
+  __shared__ __align__(16) float2 tile[1 << (R + TB)];
+  __shared__ float red[((1 << TB) / 64) * (R + TB + 1)];
+  for (int i = blockIdx.x; i < nitems; i += gridDim.x) {
+    const PassArgs &A = descs[items[2 * i]];
+    const int64_t t = items[2 * i + 1];
+    if (A.multi)
+      pass_tile<R, TB, true>(A, t, tile, red);
+    else
+      pass_tile<R, TB, false>(A, t, tile, red);
+    __syncthreads();
+  }
+}
+
 // P[b, q] from the per-tile partials of the pass that measured.
 struct FinishArgs {
+  float *out;          // (B, n) float32
+  const float *mpart;  // partials of the measuring pass
+  int64_t B;
+  int n;
+  int K;
+  int multi;
+  int nn;
   int32_t loc_of_pos[QS_MAX_QUBITS];  // local bit index of a global position, -1 if not local
   int32_t nl_of_pos[QS_MAX_QUBITS];   // index into the tile number's bits, -1 if local
 };
 
-__global__ void __launch_bounds__(256)
-k_measure_finish(float *out, const float *mpart, int n, int K, int multi, int nn, FinishArgs F) {
-  __shared__ double red[4];
-  const int64_t b = blockIdx.x;
-  const int q = blockIdx.y;
+__device__ __forceinline__ void measure_finish_block(const FinishArgs &F, int64_t b, int q, double *red) {
+  const int n = F.n, K = F.K;
   const int pos = n - 1 - q;
   double acc = 0.0;
-  if (multi) {
+  if (F.multi) {
     if (threadIdx.x == 0) {
       const int64_t t = b >> (K - n), s = b & ((int64_t(1) << (K - n)) - 1);
-      acc = mpart[((t << (K - n)) + s) * (K + 1) + pos];
+      acc = F.mpart[((t << (K - n)) + s) * (K + 1) + pos];
     }
   } else {
-    const int64_t ntl = int64_t(1) << nn;
+    const int64_t ntl = int64_t(1) << F.nn;
     const int lp = F.loc_of_pos[pos], np = F.nl_of_pos[pos];
     for (int64_t tl = threadIdx.x; tl < ntl; tl += 256) {
-      const float *pp = mpart + (b * ntl + tl) * (K + 1);
+      const float *pp = F.mpart + (b * ntl + tl) * (K + 1);
       if (lp >= 0)
         acc += pp[lp];
       else if (!((tl >> np) & 1))
@@ -267,9 +297,24 @@ k_measure_finish(float *out, const float *mpart, int n, int K, int multi, int nn
   }
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  __syncthreads();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
-  if (threadIdx.x == 0) out[b * n + q] = (float)(red[0] + red[1] + red[2] + red[3]);
+  if (threadIdx.x == 0) F.out[b * n + q] = (float)(red[0] + red[1] + red[2] + red[3]);
+}
+
+__global__ void __launch_bounds__(256) k_measure_finish(FinishArgs F) {
+  __shared__ double red[4];
+  measure_finish_block(F, blockIdx.x, blockIdx.y, red);
+}
+
+// one block per (circuit, qubit); every sample of the circuit in turn
+__global__ void __launch_bounds__(256) k_measure_finish_items(const FinishArgs *descs) {
+  __shared__ double red[4];
+  const FinishArgs &F = descs[blockIdx.x];
+  const int q = blockIdx.y;
+  if (q >= F.n) return;
+  for (int64_t b = 0; b < F.B; ++b) measure_finish_block(F, b, q, red);
 }
 
 // (B, n, 2) product-state factors of an AngleEmbedding (or |0> for unembedded bits)
@@ -306,6 +351,31 @@ __global__ void k_product_factors(float *efac, const float *x, int64_t B, int n,
   }
 }
 
+// Factors of many product states at once: entry e (one bit position of one
+// sample of one circuit) reads x[entries[2e]] (or is |0> when that index is
+// -1) with rotation entries[2e + 1].
+__global__ void k_factor_entries(float *efac, const float *x, const int32_t *entries, int64_t count) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int xi = entries[2 * e], rot = entries[2 * e + 1];
+    float2 f0 = make_float2(1.f, 0.f), f1 = make_float2(0.f, 0.f);
+    if (xi >= 0) {
+      float s, c;
+      sincosf(x[xi] * 0.5f, &s, &c);
+      if (rot == QS_RX) {
+        f0 = make_float2(c, 0.f);
+        f1 = make_float2(0.f, -s);
+      } else if (rot == QS_RY) {
+        f0 = make_float2(c, 0.f);
+        f1 = make_float2(s, 0.f);
+      } else {
+        f0 = make_float2(c, -s);
+      }
+    }
+    reinterpret_cast<float4 *>(efac)[e] = make_float4(f0.x, f0.y, f1.x, f1.y);
+  }
+}
+
 template <int R, int TB>
 static int launch_pass(const PassArgs &A, int64_t ntiles, hipStream_t st) {
   if (A.multi)
@@ -328,7 +398,7 @@ static int64_t fused_tiles(int64_t B, int n, int K) {
 extern "C" {
 
 int qs_fused_supported(int R, int TB) {
-  return (TB == 8 && R >= 1 && R <= 5) || (TB == 7 && R == 5) || (TB == 6 && R == 5);
+  return (TB == 8 && R >= 1 && R <= 5) || (TB == 7 && (R == 5 || R == 4)) || (TB == 6 && R == 5);
 }
 
 int64_t qs_fused_measure_work_bytes(int64_t B, int n, int R, int TB) {
@@ -338,14 +408,17 @@ int64_t qs_fused_measure_work_bytes(int64_t B, int n, int R, int TB) {
   return fused_tiles(B, n, K) * spt * (K + 1) * (int64_t)sizeof(float);
 }
 
-int qs_fused_pass(float *state, const float *src, int64_t B, int n, int R, int TB,
-                  const int32_t *lpos, const int32_t *words, int nsub, const float *cs,
-                  const float *efac, float *mpart, int store, void *stream) {
+}  // extern "C"
+
+// Fills a PassArgs from the public arguments (shared by the direct launch and
+// the work-item descriptors).  Returns QS_OK or QS_EINVAL.
+static int fill_pass_args(PassArgs &A, float *state, const float *src, int64_t B, int n, int R, int TB,
+                          const int32_t *lpos, const int32_t *words, int nsub, const float *cs,
+                          const float *efac, float *mpart, int store) {
   if (B < 1 || n < 1 || n > QS_MAX_QUBITS || !state || !lpos || !words || nsub < 1 ||
       !qs_fused_supported(R, TB) || R > n || (!src && !efac))
     return QS_EINVAL;
   const int K = R + TB;
-  PassArgs A;
   A.state = state;
   A.src = src;
   A.cs = cs;
@@ -376,6 +449,18 @@ int qs_fused_pass(float *state, const float *src, int64_t B, int n, int R, int T
       if (!((seen >> p) & 1)) A.npos[c++] = p;
     A.nn = c;
   }
+  return QS_OK;
+}
+
+extern "C" {
+
+int qs_fused_pass(float *state, const float *src, int64_t B, int n, int R, int TB,
+                  const int32_t *lpos, const int32_t *words, int nsub, const float *cs,
+                  const float *efac, float *mpart, int store, void *stream) {
+  PassArgs A;
+  int st0 = fill_pass_args(A, state, src, B, n, R, TB, lpos, words, nsub, cs, efac, mpart, store);
+  if (st0) return st0;
+  const int K = R + TB;
   const int64_t ntiles = fused_tiles(B, n, K);
   if (ntiles > 0x7fffffff) return QS_EUNSUPPORTED;
   hipStream_t st = (hipStream_t)stream;
@@ -389,21 +474,74 @@ int qs_fused_pass(float *state, const float *src, int64_t B, int n, int R, int T
     }
   }
   if (TB == 7 && R == 5) return launch_pass<5, 7>(A, ntiles, st);
+  if (TB == 7 && R == 4) return launch_pass<4, 7>(A, ntiles, st);
   if (TB == 6 && R == 5) return launch_pass<5, 6>(A, ntiles, st);
   return QS_EUNSUPPORTED;
 }
 
-int qs_fused_measure_finish(float *out, const float *mpart, int64_t B, int n, int R, int TB,
-                            const int32_t *lpos, void *stream) {
-  if (B < 1 || n < 1 || n > QS_MAX_QUBITS || !out || !mpart || !lpos ||
-      !qs_fused_supported(R, TB))
+int qs_factor_entries(float *efac, const float *x, const int32_t *entries, int64_t count,
+                      void *stream) {
+  if (count < 0 || (count > 0 && (!efac || !entries))) return QS_EINVAL;
+  if (count == 0) return QS_OK;
+  int64_t blocks = (count + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_factor_entries, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     efac, x, entries, count);
+  return launch_status();
+}
+
+int64_t qs_pass_desc_bytes(void) { return (int64_t)sizeof(PassArgs); }
+
+int qs_pass_desc(void *desc, int64_t *ntiles, float *state, const float *src, int64_t B, int n,
+                 int R, int TB, const int32_t *lpos, const int32_t *words, int nsub, const float *cs,
+                 const float *efac, float *mpart, int store) {
+  if (!desc || !ntiles) return QS_EINVAL;
+  PassArgs A;
+  int st0 = fill_pass_args(A, state, src, B, n, R, TB, lpos, words, nsub, cs, efac, mpart, store);
+  if (st0) return st0;
+  *ntiles = fused_tiles(B, n, R + TB);
+  memcpy(desc, &A, sizeof(PassArgs));
+  return QS_OK;
+}
+
+int qs_items_pass(const void *descs, const int32_t *items, int nitems, int R, int TB, int grid,
+                  void *stream) {
+  if (!descs || !items || nitems < 0 || grid < 1) return QS_EINVAL;
+  if (nitems == 0) return QS_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const PassArgs *d = (const PassArgs *)descs;
+  const unsigned g = (unsigned)(grid < nitems ? grid : nitems);
+  if (R == 4 && TB == 7) {
+    hipLaunchKernelGGL((k_pass_items<4, 7>), dim3(g), dim3(128), 0, st, d, items, nitems);
+    return launch_status();
+  }
+  if (R == 4 && TB == 8) {
+    hipLaunchKernelGGL((k_pass_items<4, 8>), dim3(g), dim3(256), 0, st, d, items, nitems);
+    return launch_status();
+  }
+  if (R == 5 && TB == 6) {
+    hipLaunchKernelGGL((k_pass_items<5, 6>), dim3(g), dim3(64), 0, st, d, items, nitems);
+    return launch_status();
+  }
+  return QS_EUNSUPPORTED;
+}
+
+}  // extern "C"
+
+static int fill_finish_args(FinishArgs &F, float *out, const float *mpart, int64_t B, int n, int R,
+                            int TB, const int32_t *lpos) {
+  if (B < 1 || n < 1 || n > QS_MAX_QUBITS || !out || !mpart || !lpos || !qs_fused_supported(R, TB))
     return QS_EINVAL;
   const int K = R + TB;
-  const int multi = n < K;
-  FinishArgs F;
+  F.out = out;
+  F.mpart = mpart;
+  F.B = B;
+  F.n = n;
+  F.K = K;
+  F.multi = n < K;
   for (int p = 0; p < QS_MAX_QUBITS; ++p) F.loc_of_pos[p] = F.nl_of_pos[p] = -1;
   int nn = 0;
-  if (multi) {
+  if (F.multi) {
     for (int p = 0; p < n; ++p) F.loc_of_pos[p] = p;
   } else {
     for (int j = 0; j < K; ++j) {
@@ -413,9 +551,40 @@ int qs_fused_measure_finish(float *out, const float *mpart, int64_t B, int n, in
     for (int p = 0; p < n; ++p)
       if (F.loc_of_pos[p] < 0) F.nl_of_pos[p] = nn++;
   }
+  F.nn = nn;
+  return QS_OK;
+}
+
+extern "C" {
+
+int qs_fused_measure_finish(float *out, const float *mpart, int64_t B, int n, int R, int TB,
+                            const int32_t *lpos, void *stream) {
+  FinishArgs F;
+  int st0 = fill_finish_args(F, out, mpart, B, n, R, TB, lpos);
+  if (st0) return st0;
   if (B > 0x7fffffff) return QS_EUNSUPPORTED;
   hipLaunchKernelGGL(k_measure_finish, dim3((unsigned)B, (unsigned)n), dim3(256), 0,
-                     (hipStream_t)stream, out, mpart, n, K, multi, nn, F);
+                     (hipStream_t)stream, F);
+  return launch_status();
+}
+
+int64_t qs_measure_desc_bytes(void) { return (int64_t)sizeof(FinishArgs); }
+
+int qs_measure_desc(void *desc, float *out, const float *mpart, int64_t B, int n, int R, int TB,
+                    const int32_t *lpos) {
+  if (!desc) return QS_EINVAL;
+  FinishArgs F;
+  int st0 = fill_finish_args(F, out, mpart, B, n, R, TB, lpos);
+  if (st0) return st0;
+  memcpy(desc, &F, sizeof(FinishArgs));
+  return QS_OK;
+}
+
+int qs_items_measure_finish(const void *descs, int ndesc, int max_n, void *stream) {
+  if (!descs || ndesc < 0 || max_n < 1 || max_n > QS_MAX_QUBITS) return QS_EINVAL;
+  if (ndesc == 0) return QS_OK;
+  hipLaunchKernelGGL(k_measure_finish_items, dim3((unsigned)ndesc, (unsigned)max_n), dim3(256), 0,
+                     (hipStream_t)stream, (const FinishArgs *)descs);
   return launch_status();
 }
 

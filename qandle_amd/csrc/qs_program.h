@@ -16,6 +16,11 @@
 //   QS_OP_CNOT                    target register bit a, control register bit b
 //   QS_OP_CNOT_UNIFORM            target register bit a, control = global bit
 //                                 position b that is not a register bit
+//   QS_OP_PAULI_GRAD              adjoint kernel only: reduce Im<lam|P|psi> into
+//                                 gradient row b without changing the state;
+//                                 the coef word holds the Pauli (0 X, 1 Y, 2 Z)
+//                                 plus QS_PAULI_UNIFORM when a is a global
+//                                 position that is not a register bit (Z only)
 // Coefficients are float2 (cos(phi/2), sin(phi/2)) entries of the `cs` table.
 #pragma once
 
@@ -25,7 +30,9 @@
 #define QS_OP_RZ_UNIFORM 3
 #define QS_OP_CNOT 4
 #define QS_OP_CNOT_UNIFORM 5
+#define QS_OP_PAULI_GRAD 6
 #define QS_OP_PER_SAMPLE 0x100
+#define QS_PAULI_UNIFORM 0x10
 
 #define QS_MAX_LOCAL_BITS 16
 #define QS_MAX_QUBITS 40
