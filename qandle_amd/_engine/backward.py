@@ -125,9 +125,10 @@ class FusedCircuitFn(torch.autograd.Function):
         g_param = None
         if ctx.needs_input_grad[6]:
             param_rows = []
+            slot_pos = {s_: i for i, s_ in enumerate(prog.param_slots)}
             for gi, g in enumerate(prog.gates):
                 if g.kind != 4 and prog.slots[g.slot].kind == "param":
-                    param_rows.append((prog.param_slots.index(g.slot), rows[gi]))
+                    param_rows.append((slot_pos[g.slot], rows[gi]))
             g_param = torch.zeros(len(prog.param_slots), dtype=torch.float32, device=psi.device)
             if param_rows:
                 idx = torch.tensor([p for p, _ in param_rows], device=psi.device)

@@ -41,13 +41,9 @@ class Program:
     plans: dict = dataclasses.field(default_factory=dict)
     words_cache: dict = dataclasses.field(default_factory=dict)
 
-    @property
-    def param_slots(self):
-        return [i for i, s in enumerate(self.slots) if s.kind == "param"]
-
-    @property
-    def named_slots(self):
-        return [i for i, s in enumerate(self.slots) if s.kind == "named"]
+    def __post_init__(self):
+        self.param_slots = [i for i, s in enumerate(self.slots) if s.kind == "param"]
+        self.named_slots = [i for i, s in enumerate(self.slots) if s.kind == "named"]
 
 
 def _lower_into(layers, n, mapping, prog_gates, slots, state):
