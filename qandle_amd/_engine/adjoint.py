@@ -26,12 +26,12 @@ def _needs_grad(inp) -> bool:
 
 
 def forward(prog, inp):
-    R, TB = fused.choose_shape(prog.n)
+    R, TB, cm = fused.choose_config(prog, inp)
     if not _needs_grad(inp):
         cs = fused.build_coeffs(prog, inp)
         efac = fused.product_factors(prog, inp) if (prog.init[0] == "embed" or inp.state is None) else None
-        psi, probs = fused.execute(prog, inp, cs, efac, R, TB, want_state=not prog.measure)
+        psi, probs = fused.execute(prog, inp, cs, efac, R, TB, cm=cm, want_state=not prog.measure)
         return psi, probs
     from . import backward
 
-    return backward.run_with_grad(prog, inp, R, TB)
+    return backward.run_with_grad(prog, inp, R, TB, cm)

@@ -48,14 +48,14 @@ def default_checkpoint_gates(n: int) -> int:
 
 class FusedCircuitFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, prog, R, TB, B, batched, names, phi_param, embed_x, state, *named_vals):
+    def forward(ctx, prog, R, TB, cm, B, batched, names, phi_param, embed_x, state, *named_vals):
         named = dict(zip(names, [v.detach() for v in named_vals]))
         inp = fused.Inputs(B, batched, phi_param.device, phi_param.detach(), named,
                            None if embed_x is None else embed_x.detach(),
                            None if state is None else state.detach())
         cs = fused.build_coeffs(prog, inp)
         efac = fused.product_factors(prog, inp) if (prog.init[0] == "embed" or state is None) else None
-        pl = fused.get_plan(prog, R, TB)
+        pl = fused.get_plan(prog, R, TB, cm)
         every = config.CHECKPOINT_GATES if getattr(config, "CHECKPOINT_GATES", None) else default_checkpoint_gates(prog.n)
         marks = checkpoint_after(prog, pl, every)
         ckpts = {}
@@ -64,8 +64,8 @@ class FusedCircuitFn(torch.autograd.Function):
             if i in marks:
                 ckpts[i] = psi.clone()
 
-        psi, probs = fused.execute(prog, inp, cs, efac, R, TB, want_state=True, checkpoints=keep)
-        ctx.prog, ctx.R, ctx.TB, ctx.B, ctx.names = prog, R, TB, B, names
+        psi, probs = fused.execute(prog, inp, cs, efac, R, TB, cm=cm, want_state=True, checkpoints=keep)
+        ctx.prog, ctx.R, ctx.TB, ctx.cm, ctx.B, ctx.names = prog, R, TB, cm, B, names
         ctx.ck_index = sorted(ckpts)
         ctx.named_shapes = [v.shape for v in named_vals]
         ctx.embed_shape = None if embed_x is None else embed_x.shape
@@ -92,14 +92,14 @@ class FusedCircuitFn(torch.autograd.Function):
             seed = gout.to(torch.float32).contiguous()
         else:
             lam.copy_(gout.to(torch.complex64).resolve_conj())
-        pl = fused.get_plan(prog, R, TB)
+        pl = fused.get_plan(prog, R, TB, ctx.cm)
         words = fused.pass_words(prog, pl, B, psi.device, reverse=True)
         rows = fused.rotation_rows(prog)
         nrows = len(rows)
         embed_rows = 0
         gwords = []
-        if prog.init[0] == "embed" and ctx.needs_input_grad[7]:
-            gplan, gwords = embed_grad_words(prog, R, TB, B, psi.device, nrows)
+        if prog.init[0] == "embed" and ctx.needs_input_grad[8]:
+            gplan, gwords = embed_grad_words(prog, R, TB, B, psi.device, nrows, ctx.cm)
             embed_rows = len(prog.init[2])
         total_rows = nrows + embed_rows
         wb = lib.qs_adjoint_work_bytes(B, n, R, TB, total_rows)
@@ -123,7 +123,7 @@ class FusedCircuitFn(torch.autograd.Function):
 
         # parameters (shared angles): one row each
         g_param = None
-        if ctx.needs_input_grad[6]:
+        if ctx.needs_input_grad[7]:
             param_rows = []
             slot_pos = {s_: i for i, s_ in enumerate(prog.param_slots)}
             for gi, g in enumerate(prog.gates):
@@ -137,7 +137,7 @@ class FusedCircuitFn(torch.autograd.Function):
         # named inputs: per-sample rows summed over the gates that use the name
         g_named = []
         for k, name in enumerate(ctx.names):
-            if not ctx.needs_input_grad[9 + k]:
+            if not ctx.needs_input_grad[10 + k]:
                 g_named.append(None)
                 continue
             rr = [rows[gi] for gi, g in enumerate(prog.gates) if g.kind != 4 and prog.slots[g.slot].kind == "named"
@@ -151,17 +151,17 @@ class FusedCircuitFn(torch.autograd.Function):
             shp = ctx.embed_shape
             g_embed = gx.sum(0) if len(shp) == 1 else gx.reshape(shp)
         g_state = None
-        if ctx.needs_input_grad[8] and prog.init[0] != "embed":
+        if ctx.needs_input_grad[9] and prog.init[0] != "embed":
             shp = ctx.state_shape
             g_state = lam.sum(0) if len(shp) == 1 else (lam if lam.shape == shp else lam.sum(0, keepdim=True))
-        return (None, None, None, None, None, None, g_param, g_embed, g_state, *g_named)
+        return (None, None, None, None, None, None, None, g_param, g_embed, g_state, *g_named)
 
 
-def embed_grad_words(prog, R, TB, B, device, row0):
+def embed_grad_words(prog, R, TB, B, device, row0, cm=None):
     """Adjoint passes that only reduce 1/2 Im<lam|P_j|psi> for every embedded bit
     (rows row0 + j); the factors of the product state commute, so one pair
     (psi_0, lam_0) gives every column."""
-    key = ("embed_grad", R, TB, B, str(device), row0)
+    key = ("embed_grad", R, TB, B, str(device), row0, None if cm is None else (cm.budget, cm.min_low))
     hit = prog.words_cache.get(key)
     if hit is not None:
         return hit
@@ -169,7 +169,7 @@ def embed_grad_words(prog, R, TB, B, device, row0):
 
     _, rot, bits, _ = prog.init
     gates = [PL.Gate(PL.PGRAD + rot, b, -1, j) for j, b in enumerate(bits)]
-    gplan = PL.plan(prog.n, gates, R, TB, fused.DEFAULT_COST)
+    gplan = PL.plan(prog.n, gates, R, TB, cm or fused.DEFAULT_COST)
     out = []
     for p in gplan.passes:
         w = PL.encode_pass(p, gates, R, TB, lambda s_: 0, lambda s_: False, reverse=True,
@@ -179,11 +179,11 @@ def embed_grad_words(prog, R, TB, B, device, row0):
     return gplan, out
 
 
-def run_with_grad(prog, inp, R, TB):
+def run_with_grad(prog, inp, R, TB, cm=None):
     names = list(inp.named)
     vals = [inp.named[k] for k in names]
-    out = FusedCircuitFn.apply(prog, R, TB, inp.B, inp.batched, names, inp.phi_param, inp.embed_x,
-                               inp.state, *vals)
+    out = FusedCircuitFn.apply(prog, R, TB, cm or fused.DEFAULT_COST, inp.B, inp.batched, names, inp.phi_param,
+                               inp.embed_x, inp.state, *vals)
     if prog.measure:
         return None, out
     return out, None

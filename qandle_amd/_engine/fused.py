@@ -26,9 +26,10 @@ DEFAULT_R = 5
 class Slot:
     """Where a rotation's angle comes from."""
 
-    kind: str                     # "param" (remap(theta) of a module) or "named"
+    kind: str                     # "param" (remap(theta) of a module), "named", or "value" (a fixed angle)
     module: typing.Any = None
     name: str = ""
+    value: float = 0.0
 
 
 @dataclasses.dataclass
@@ -42,7 +43,7 @@ class Program:
     words_cache: dict = dataclasses.field(default_factory=dict)
 
     def __post_init__(self):
-        self.param_slots = [i for i, s in enumerate(self.slots) if s.kind == "param"]
+        self.param_slots = [i for i, s in enumerate(self.slots) if s.kind in ("param", "value")]
         self.named_slots = [i for i, s in enumerate(self.slots) if s.kind == "named"]
 
 
@@ -118,6 +119,29 @@ def lower_circuit(circuit) -> typing.Optional[Program]:
     return prog
 
 
+def program_from_spec(n: int, gates, embed=None, measure: bool = True) -> Program:
+    """Program straight from a gate list, without building operator modules.
+
+    ``gates``: (kind, qubit, angle) with kind in rx/ry/rz, or ("cnot", (c, t));
+    ``embed``: (rot in "xyz", qubits, input name) or None (start from |0..0>).
+    Angles are fixed values (no remapping)."""
+    kinds = {"rx": PL.RX, "ry": PL.RY, "rz": PL.RZ}
+    rot_kind = {"x": 0, "y": 1, "z": 2}
+    pg, slots = [], []
+    for g in gates:
+        if g[0] == "cnot":
+            c, t = g[1]
+            pg.append(PL.Gate(PL.CNOT, n - 1 - t, n - 1 - c))
+        else:
+            slots.append(Slot("value", value=float(g[2])))
+            pg.append(PL.Gate(kinds[g[0]], n - 1 - g[1], -1, len(slots) - 1))
+    init = ("input",)
+    if embed is not None:
+        rot, qubits, name = embed
+        init = ("embed", rot_kind[rot], [n - 1 - q for q in qubits], name)
+    return Program(n, pg, slots, init, measure)
+
+
 def get_plan(prog: Program, R: int, TB: int, cm: PL.CostModel = None) -> PL.Plan:
     cm = cm or DEFAULT_COST
     key = (R, TB, None if cm is None else dataclasses.astuple(cm))
@@ -129,6 +153,22 @@ def get_plan(prog: Program, R: int, TB: int, cm: PL.CostModel = None) -> PL.Plan
 
 
 DEFAULT_COST = PL.CostModel(budget=160.0, min_low=4)
+
+
+def choose_config(prog: Program, inp):
+    """(R, TB, cost model) for a call: the autotuned choice when enabled, else defaults."""
+    from .. import config
+
+    if getattr(config, "AUTOTUNE", False):
+        from . import autotune
+
+        hit = autotune.lookup(prog, inp.B, inp.device)
+        if hit is None:
+            with torch.no_grad():
+                hit, _ = autotune.tune(prog, inp)
+        return hit
+    R, TB = choose_shape(prog.n)
+    return R, TB, DEFAULT_COST
 
 
 def choose_shape(n: int):
@@ -224,9 +264,10 @@ def gather_inputs(prog: Program, circuit, state, kwargs) -> Inputs:
                 named[name] = v.expand(B)
             else:
                 raise RuntimeError(f"named input '{name}' has {v.shape[0]} entries, batch is {B}")
-    params = [prog.slots[i].module for i in prog.param_slots]
+    params = [prog.slots[i] for i in prog.param_slots]
     if params:
-        phis = [m.remapping(m.theta).reshape(-1)[:1].to(device=device, dtype=torch.float32) for m in params]
+        phis = [(s.module.remapping(s.module.theta).reshape(-1)[:1] if s.kind == "param"
+                 else torch.tensor([s.value])).to(device=device, dtype=torch.float32) for s in params]
         phi_param = torch.cat(phis)
     else:
         phi_param = torch.zeros(0, dtype=torch.float32, device=device)

@@ -68,6 +68,12 @@ class UnevenBatch:
     remapped) at construction; ``set_angles`` refreshes them.
     """
 
+    @classmethod
+    def from_workloads(cls, workloads, device, **kw):
+        """Batch straight from gate-list workloads (no operator modules are built)."""
+        progs = [fused.program_from_spec(w.n, w.gates, w.embed) for w in workloads]
+        return cls(progs, [w.inputs[w.embed[2]] for w in workloads], device, **kw)
+
     def __init__(self, circuits, inputs, device, R: int = 4, TB: int = 7, grid: int = 2048,
                  cost: PL.CostModel = None):
         self.device = torch.device(device)
@@ -80,7 +86,7 @@ class UnevenBatch:
         off = dict(phi=0, x=0, state=0, out=0, mpart=0, gpart=0, gout=0, gtot=0, efac=0)
         self.modules = circuits
         for ci, (circ, x) in enumerate(zip(circuits, inputs)):
-            prog = fused.lower_circuit(circ.circuit)
+            prog = circ if isinstance(circ, fused.Program) else fused.lower_circuit(circ.circuit)
             if prog is None or not prog.measure or prog.named_slots:
                 raise ValueError(f"circuit {ci}: uneven batches take embedding + gates + MeasureProbability "
                                  "circuits without named rotation inputs")
@@ -111,9 +117,11 @@ class UnevenBatch:
                       off["state"], off["out"], off["mpart"], off["gpart"], off["gout"], off["gtot"],
                       off["efac"], rows, nrot)
             self.circs.append(c)
-            params = [prog.slots[i].module for i in prog.param_slots]
+            params = [prog.slots[i] for i in prog.param_slots]
             if params:
-                phi_vals.append(torch.cat([m.remapping(m.theta).detach().reshape(-1)[:1].float().cpu() for m in params]))
+                phi_vals.append(torch.tensor(
+                    [float(sl.module.remapping(sl.module.theta).detach().reshape(-1)[0]) if sl.kind == "param"
+                     else sl.value for sl in params], dtype=torch.float32))
             if d:
                 x_vals.append(x.reshape(-1))
             off["phi"] += c.nphi

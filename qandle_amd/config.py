@@ -11,3 +11,5 @@ DEFAULT_MAPPING = qw_map.tanh
 FUSED = True            # Circuit.forward uses the fused pass engine
 CHECKPOINT_LAYERS = 5   # adjoint backward restores psi from a checkpoint every k layers
 CHECKPOINT_GATES = None  # explicit checkpoint spacing in gates (overrides CHECKPOINT_LAYERS)
+AUTOTUNE = False         # time the kernel shape / fusion budget candidates on first use per (circuit, batch)
+AUTOTUNE_CACHE = None    # JSON file that keeps tuned choices across processes

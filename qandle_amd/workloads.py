@@ -12,6 +12,7 @@ import dataclasses
 import math
 import typing
 
+import numpy as np
 import torch
 
 from . import embeddings, measurements, operators
@@ -105,24 +106,31 @@ def rxryrz_ladder(n=20, B=64, depth=10, seed=1, loss=False):
     return w
 
 
-def random_uneven(num_circuits=2048, n_lo=4, n_hi=18, d_lo=2, d_hi=24, p_cnot=0.3, seed=3):
+def random_uneven(num_circuits=2048, n_lo=4, n_hi=18, d_lo=2, d_hi=24, p_cnot=0.3, seed=3, build_layers=True):
     """Circuits of uneven width and depth: per layer one random gate from {RX, RY, RZ}
-    per qubit (angles U[0, 2pi)) and a CNOT on each adjacent pair with probability p."""
-    gen = torch.Generator().manual_seed(seed)
+    per qubit (angles U[0, 2pi)) and a CNOT on each adjacent pair q -> q+1 with
+    probability p; batch 1; RX AngleEmbedding with inputs U[0, pi)."""
+    rng = np.random.default_rng(seed)
+    kinds = ("rx", "ry", "rz")
     out = []
     for c in range(num_circuits):
-        n = int(torch.randint(n_lo, n_hi + 1, (), generator=gen))
-        depth = int(torch.randint(d_lo, d_hi + 1, (), generator=gen))
+        n = int(rng.integers(n_lo, n_hi + 1))
+        depth = int(rng.integers(d_lo, d_hi + 1))
         body = []
         for _ in range(depth):
+            ks = rng.integers(0, 3, n)
+            th = rng.uniform(0.0, 2 * math.pi, n).astype(np.float32)
             for q in range(n):
-                k = ("rx", "ry", "rz")[int(torch.randint(0, 3, (), generator=gen))]
-                body.append((k, q, float(_uniform(gen, (), 0.0, 2 * math.pi))))
+                body.append((kinds[ks[q]], q, float(th[q])))
+            coin = rng.random(max(n - 1, 0))
             for q in range(n - 1):
-                if float(torch.rand((), generator=gen)) < p_cnot:
+                if coin[q] < p_cnot:
                     body.append(("cnot", (q, q + 1)))
-        x = _uniform(gen, (1, n), 0.0, math.pi)
-        out.append(_assemble(f"uneven_{c}_n{n}_d{depth}", n, 1, x, body))
+        x = torch.from_numpy(rng.uniform(0.0, math.pi, (1, n)).astype(np.float32))
+        if build_layers:
+            out.append(_assemble(f"uneven_{c}_n{n}_d{depth}", n, 1, x, body))
+        else:
+            out.append(Workload(f"uneven_{c}_n{n}_d{depth}", n, 1, [], {"x": x}, body, ("x", tuple(range(n)), "x")))
     return out
 
 

@@ -1,0 +1,73 @@
+"""Serving loop: admission by HBM footprint (CPU, host logic only) and, on the
+GPU, identical results alone and in a concurrent mix."""
+import numpy as np
+import pytest
+import torch
+
+import qandle_amd as qandle
+from oracle import fp64
+from qandle_amd import workloads
+from qandle_amd._engine import serving
+
+
+def _req(rid, n, B, depth, seed, device="cpu", arrival=0.0):
+    gen = torch.Generator().manual_seed(seed)
+    w = workloads.hardware_efficient(n, B, depth, gen)
+    circ = qandle.Circuit(w.layers, num_qubits=n)
+    if device != "cpu":
+        circ = circ.to(device)
+    return serving.Request(rid, circ, {"x": w.inputs["x"].to(device)}, arrival=arrival), w
+
+
+def test_admission_footprint_chunking_and_rejection():
+    eng = serving.ServingEngine.__new__(serving.ServingEngine)  # host-side parts only
+    eng.cap = 10 * (2 ** 12) * 8 + 50_000
+    r, _ = _req(0, 12, 37, 2, 1)
+    eng.prepare(r)
+    assert r.error is None and r.n == 12 and r.B == 37
+    assert r.footprint > eng.cap
+    assert r.chunks[0] == (0, r.chunks[0][1]) and r.chunks[-1][1] == 37
+    sizes = [hi - lo for lo, hi in r.chunks]
+    assert all(serving.state_footprint(12, s, 5, 6) <= eng.cap for s in sizes)
+    big, _ = _req(1, 16, 1, 1, 2)
+    eng.cap = 2 ** 16 * 8 // 2
+    eng.prepare(big)
+    assert big.error is not None and "admission cap" in big.error
+
+
+def test_poisson_stream_is_seeded():
+    a = serving.poisson_requests.__wrapped__ if hasattr(serving.poisson_requests, "__wrapped__") else None
+    assert a is None  # plain function
+    rng1 = np.random.default_rng(3).exponential(1.0, 5)
+    rng2 = np.random.default_rng(3).exponential(1.0, 5)
+    assert np.array_equal(rng1, rng2)
+
+
+@pytest.mark.gpu
+def test_results_identical_alone_and_in_mix(device):
+    reqs_mix, reqs_alone, ws = [], [], []
+    spec = [(8, 5, 3), (12, 3, 4), (14, 2, 2), (10, 7, 5), (13, 4, 3), (9, 1, 2), (15, 2, 4), (11, 6, 3)] * 3
+    for i, (n, B, d) in enumerate(spec):
+        r, w = _req(i, n, B, d, 100 + i, device, arrival=0.0005 * i)
+        reqs_mix.append(r)
+        ws.append(w)
+        r2, _ = _req(i, n, B, d, 100 + i, device, arrival=0.0)
+        reqs_alone.append(r2)
+    eng = serving.ServingEngine(device, cap_bytes=64 * 2 ** 20, streams=4)
+    for r in reqs_mix:
+        eng.prepare(r)
+    done = eng.serve(reqs_mix)
+    assert len(done) == len(spec) and all(r.error is None for r in done)
+    assert eng.peak_resident <= 64 * 2 ** 20
+    for r in reqs_alone:
+        solo = serving.ServingEngine(device, cap_bytes=64 * 2 ** 20, streams=1, max_inflight=1)
+        solo.prepare(r)
+        solo.serve([r])
+    for a, b, w in zip(reqs_mix, reqs_alone, ws):
+        assert torch.equal(a.result, b.result), f"request {a.rid} differs alone vs in the mix"
+    for i in (0, 3, 6):
+        w = ws[i]
+        ref = fp64.measure_prob(fp64.simulate(w.n, w.oracle_ops()), w.n)
+        assert np.abs(reqs_mix[i].result.cpu().numpy() - ref).max() < 1e-5
+    st = serving.latency_stats(done)
+    assert st["p50_ms"] > 0 and st["p99_ms"] >= st["p50_ms"]
