@@ -121,15 +121,29 @@ int qs_product_factors(float *efac, const float *x, int64_t B, int n,
                        const int32_t *bits, int d, int rot, int64_t x_stride,
                        void *stream);
 
+/* Fused per-bit unitaries and gradient generators for a program's words:
+ * `recipes` (device int32, record r at offsets[r]) lists, per table entry,
+ * u_index, span (batch size of a per-sample entry, else 1), nconst, q_index
+ * and per fused rotation (kind, cs index, cs per-sample flag) in application
+ * order.  For b < span writes utab[u_index + b] = R_m ... R_1 and
+ * qtab[q_index + k*span + b] = V_k P_k V_k^dagger with
+ * V_k = R_m ... R_{k+1} (8 floats each; qtab = NULL skips the generators,
+ * which only the adjoint kernels read).  Replaces the per-call rebuild of
+ * every gate's dense matrix (src/qandle/operators.py:265-275). */
+int qs_compose_tables(float *utab, float *qtab, const int32_t *recipes,
+                      const int32_t *offsets, int nrec, const float *cs,
+                      void *stream);
+
 /* One pass.  src = NULL generates the product state of efac in registers;
  * otherwise tiles are read from src (src may equal state).  cs is the float2
- * (cos, sin) table the op words index.  mpart != NULL folds
+ * (cos, sin) table of the phase entries and utab the fused unitaries that the
+ * words index.  mpart != NULL folds
  * MeasureProbability partials into the pass (size from
  * qs_fused_measure_work_bytes).  store = 0 skips writing the state back. */
 int qs_fused_pass(float *state, const float *src, int64_t B, int n, int R,
                   int TB, const int32_t *lpos, const int32_t *words, int nsub,
-                  const float *cs, const float *efac, float *mpart, int store,
-                  void *stream);
+                  const float *cs, const float *utab, const float *efac,
+                  float *mpart, int store, void *stream);
 
 /* MeasureProbability from a measuring pass's partials: out (B, n) float32,
  * lpos (host) = that pass's local positions.
@@ -154,8 +168,8 @@ int qs_fused_measure_finish(float *out, const float *mpart, int64_t B, int n,
 int64_t qs_adjoint_work_bytes(int64_t B, int n, int R, int TB, int rows);
 int qs_adjoint_pass(float *psi, float *lam, int64_t B, int n, int R, int TB,
                     const int32_t *lpos, const int32_t *words, int nsub,
-                    const float *cs, float *gpart, int rows,
-                    const float *seed_w, void *stream);
+                    const float *cs, const float *utab, const float *qtab,
+                    float *gpart, int rows, const float *seed_w, void *stream);
 int qs_adjoint_finish(float *out, float *tot, const float *gpart, int rows,
                       int64_t B, int n, int R, int TB, void *stream);
 
@@ -181,7 +195,7 @@ int64_t qs_pass_desc_bytes(void);
 int qs_pass_desc(void *desc, int64_t *ntiles, float *state, const float *src,
                  int64_t B, int n, int R, int TB, const int32_t *lpos,
                  const int32_t *words, int nsub, const float *cs,
-                 const float *efac, float *mpart, int store);
+                 const float *utab, const float *efac, float *mpart, int store);
 int qs_items_pass(const void *descs, const int32_t *items, int nitems, int R,
                   int TB, int grid, void *stream);
 
@@ -195,7 +209,8 @@ int64_t qs_adjoint_desc_bytes(void);
 int qs_adjoint_desc(void *desc, int64_t *ntiles, float *psi, float *lam,
                     int64_t B, int n, int R, int TB, const int32_t *lpos,
                     const int32_t *words, int nsub, const float *cs,
-                    float *gpart, int rows, const float *seed_w);
+                    const float *utab, const float *qtab, float *gpart,
+                    int rows, const float *seed_w);
 int qs_items_adjoint(const void *descs, const int32_t *items, int nitems,
                      int R, int TB, int grid, void *stream);
 

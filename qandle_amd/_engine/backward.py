@@ -93,28 +93,28 @@ class FusedCircuitFn(torch.autograd.Function):
         else:
             lam.copy_(gout.to(torch.complex64).resolve_conj())
         pl = fused.get_plan(prog, R, TB, ctx.cm)
-        words = fused.pass_words(prog, pl, B, psi.device, reverse=True)
         rows = fused.rotation_rows(prog)
         nrows = len(rows)
-        embed_rows = 0
-        gwords = []
-        if prog.init[0] == "embed" and ctx.needs_input_grad[8]:
-            gplan, gwords = embed_grad_words(prog, R, TB, B, psi.device, nrows, ctx.cm)
-            embed_rows = len(prog.init[2])
+        embed = prog.init[0] == "embed" and ctx.needs_input_grad[8]
+        ws, nmain = fused.adjoint_words(prog, pl, B, psi.device, embed, ctx.cm)
+        embed_rows = len(prog.init[2]) if embed else 0
         total_rows = nrows + embed_rows
+        utab, qtab = fused.compose_tables(ws, cs)
         wb = lib.qs_adjoint_work_bytes(B, n, R, TB, total_rows)
         gpart = torch.empty(max(wb // 4, 1), dtype=torch.float32, device=psi.device)
-        sweep = [(words[i], i) for i in range(len(words) - 1, -1, -1)] + [(gw, None) for gw in gwords]
-        for k, ((w, nsub, lpos), i) in enumerate(sweep):
+        npass = len(pl.passes)
+        for k, (w, nsub, lpos) in enumerate(ws.passes):
             L.check(
                 lib.qs_adjoint_pass(
                     L.ptr(psi), L.ptr(lam), B, n, R, TB,
                     lpos.ctypes.data_as(L.ctypes.POINTER(L.ctypes.c_int32)), L.ptr(w), nsub,
-                    L.ptr(cs), L.ptr(gpart), total_rows, None if (k > 0 or seed is None) else L.ptr(seed), st,
+                    L.ptr(cs), L.ptr(utab), L.ptr(qtab), L.ptr(gpart), total_rows,
+                    None if (k > 0 or seed is None) else L.ptr(seed), st,
                 ),
                 "qs_adjoint_pass",
             )
-            if i is not None and (i - 1) in ckpts:
+            i = npass - 1 - k  # forward pass this step undid
+            if k < nmain and (i - 1) in ckpts:
                 psi.copy_(ckpts[i - 1])
         per = torch.empty(max(total_rows, 1), B, dtype=torch.float32, device=psi.device)
         tot = torch.empty(max(total_rows, 1), dtype=torch.float32, device=psi.device)
@@ -155,28 +155,6 @@ class FusedCircuitFn(torch.autograd.Function):
             shp = ctx.state_shape
             g_state = lam.sum(0) if len(shp) == 1 else (lam if lam.shape == shp else lam.sum(0, keepdim=True))
         return (None, None, None, None, None, None, None, g_param, g_embed, g_state, *g_named)
-
-
-def embed_grad_words(prog, R, TB, B, device, row0, cm=None):
-    """Adjoint passes that only reduce 1/2 Im<lam|P_j|psi> for every embedded bit
-    (rows row0 + j); the factors of the product state commute, so one pair
-    (psi_0, lam_0) gives every column."""
-    key = ("embed_grad", R, TB, B, str(device), row0, None if cm is None else (cm.budget, cm.min_low))
-    hit = prog.words_cache.get(key)
-    if hit is not None:
-        return hit
-    from . import planner as PL
-
-    _, rot, bits, _ = prog.init
-    gates = [PL.Gate(PL.PGRAD + rot, b, -1, j) for j, b in enumerate(bits)]
-    gplan = PL.plan(prog.n, gates, R, TB, cm or fused.DEFAULT_COST)
-    out = []
-    for p in gplan.passes:
-        w = PL.encode_pass(p, gates, R, TB, lambda s_: 0, lambda s_: False, reverse=True,
-                           row_of=lambda gi: row0 + gi)
-        out.append((torch.from_numpy(w).to(device), len(p.subs), __import__("numpy").asarray(p.L, dtype="int32")))
-    prog.words_cache[key] = (gplan, out)
-    return gplan, out
 
 
 def run_with_grad(prog, inp, R, TB, cm=None):

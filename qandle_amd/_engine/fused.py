@@ -303,20 +303,92 @@ def rotation_rows(prog: Program):
     return rows
 
 
-def pass_words(prog: Program, pl: PL.Plan, B: int, device, reverse: bool = False) -> list:
-    key = (id(pl), B, str(device), reverse)
+@dataclasses.dataclass
+class WordSet:
+    """Encoded passes plus the recipes of the fused-unitary tables they index."""
+
+    passes: list                  # [(int32 words on the device, sub-pass count, lpos int32 numpy)]
+    recipes: torch.Tensor         # int32 recipe words (qs_compose_tables)
+    offsets: torch.Tensor         # int32 start of every recipe
+    nrec: int
+    nu: int                       # U entries
+    nq: int                       # gradient-generator entries
+
+
+def upload_tables(passes, tables: PL.TableBuilder, device) -> WordSet:
+    rec = torch.from_numpy(tables.recipes()).to(device)
+    off = torch.from_numpy(np.asarray(tables.offsets or [0], dtype=np.int32)).to(device)
+    return WordSet(passes, rec, off, tables.count, tables.nu, tables.nq)
+
+
+def compose_tables(ws: WordSet, cs: torch.Tensor, generators: bool = True):
+    """(utab, qtab): the fused 2x2 unitaries and (for the adjoint) their gradient
+    generators for this call's angles; qtab is None without ``generators``."""
+    utab = torch.empty(max(ws.nu, 1) * 8, dtype=torch.float32, device=cs.device)
+    qtab = torch.empty(max(ws.nq, 1) * 8, dtype=torch.float32, device=cs.device) if generators else None
+    L.check(L.load().qs_compose_tables(L.ptr(utab), None if qtab is None else L.ptr(qtab), L.ptr(ws.recipes),
+                                       L.ptr(ws.offsets), ws.nrec, L.ptr(cs), L.stream_of(cs)),
+            "qs_compose_tables")
+    return utab, qtab
+
+
+def _encode(p, gates, pl, index, per_sample, tables, rows=None):
+    w = PL.encode_pass_v2(p, gates, pl.R, pl.TB, index, per_sample, tables,
+                          reverse=rows is not None, row_of=rows)
+    return w, len(p.subs), np.asarray(p.L, dtype=np.int32)
+
+
+def pass_words(prog: Program, pl: PL.Plan, B: int, device) -> WordSet:
+    """Forward words of every pass (cached per plan, batch and device)."""
+    key = (id(pl), B, str(device), "fwd")
     cached = prog.words_cache.get(key)
     if cached is not None:
         return cached
     index, per_sample, _, _ = coefficient_layout(prog, B)
-    rows = rotation_rows(prog) if reverse else None
+    tables = PL.TableBuilder(B)
     out = []
     for p in pl.passes:
-        w = PL.encode_pass(p, prog.gates, pl.R, pl.TB, lambda s: index[s], lambda s: per_sample[s],
-                           reverse=reverse, row_of=(rows.__getitem__ if reverse else None))
-        out.append((torch.from_numpy(w).to(device), len(p.subs), np.asarray(p.L, dtype=np.int32)))
-    prog.words_cache[key] = out
-    return out
+        w, nsub, lpos = _encode(p, prog.gates, pl, index.__getitem__, per_sample.__getitem__, tables)
+        out.append((torch.from_numpy(w).to(device), nsub, lpos))
+    ws = upload_tables(out, tables, device)
+    prog.words_cache[key] = ws
+    return ws
+
+
+def embed_grad_gates(prog: Program):
+    """Gradient-only probes, one per embedded bit (Pauli of the embedding rotation)."""
+    _, rot, bits, _ = prog.init
+    return [PL.Gate(PL.PGRAD + rot, b, -1, j) for j, b in enumerate(bits)]
+
+
+def adjoint_words(prog: Program, pl: PL.Plan, B: int, device, embed: bool, cm: PL.CostModel = None):
+    """Reverse-sweep words: the passes in reverse (rotation rows in program
+    order), then, with ``embed``, the probe passes of the embedded bits (rows
+    after the rotations).  Returns (WordSet, number of reversed forward passes)."""
+    key = (id(pl), B, str(device), "adj", embed)
+    cached = prog.words_cache.get(key)
+    if cached is not None:
+        return cached
+    index, per_sample, _, _ = coefficient_layout(prog, B)
+    rows = rotation_rows(prog)
+    tables = PL.TableBuilder(B)
+    out = []
+    for p in reversed(pl.passes):
+        w, nsub, lpos = _encode(p, prog.gates, pl, index.__getitem__, per_sample.__getitem__, tables,
+                                rows.__getitem__)
+        out.append((torch.from_numpy(w).to(device), nsub, lpos))
+    nmain = len(out)
+    if embed:
+        gates = embed_grad_gates(prog)
+        gplan = PL.plan(prog.n, gates, pl.R, pl.TB, cm or DEFAULT_COST)
+        row0 = len(rows)
+        for p in gplan.passes:
+            w, nsub, lpos = _encode(p, gates, gplan, lambda s_: 0, lambda s_: False, tables,
+                                    lambda gi: row0 + gi)
+            out.append((torch.from_numpy(w).to(device), nsub, lpos))
+    hit = (upload_tables(out, tables, device), nmain)
+    prog.words_cache[key] = hit
+    return hit
 
 
 def build_coeffs(prog: Program, inp: Inputs) -> torch.Tensor:
@@ -365,9 +437,11 @@ def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
     launch stream around every pass kernel is appended to it."""
     n, B = prog.n, inp.B
     pl = get_plan(prog, R, TB, cm)
-    words = pass_words(prog, pl, B, inp.device)
+    ws = pass_words(prog, pl, B, inp.device)
+    words = ws.passes
     lib = L.load()
     st = L.stream_of(cs)
+    utab, _ = compose_tables(ws, cs, generators=False)
     psi = torch.empty(B, 2 ** n, dtype=torch.complex64, device=inp.device)
     src = None
     if prog.init[0] == "input" and inp.state is not None:
@@ -392,7 +466,7 @@ def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
             lib.qs_fused_pass(
                 L.ptr(psi), None if src is None else L.ptr(src), B, n, R, TB,
                 lpos.ctypes.data_as(L.ctypes.POINTER(L.ctypes.c_int32)), L.ptr(w), nsub,
-                L.ptr(cs), None if efac is None else L.ptr(efac),
+                L.ptr(cs), L.ptr(utab), None if efac is None else L.ptr(efac),
                 L.ptr(mpart) if (last and mpart is not None) else None, store, st,
             ),
             "qs_fused_pass",

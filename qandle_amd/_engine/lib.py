@@ -40,21 +40,22 @@ SIGNATURES = {
     "qs_pauli_im": (I32, [P, P, P, I64, I32, P, I32, I32, P, I64, P]),
     "qs_fused_supported": (I32, [I32, I32]),
     "qs_product_factors": (I32, [P, P, I64, I32, P, I32, I32, I64, P]),
-    "qs_fused_pass": (I32, [P, P, I64, I32, I32, I32, P, P, I32, P, P, P, I32, P]),
+    "qs_fused_pass": (I32, [P, P, I64, I32, I32, I32, P, P, I32, P, P, P, P, I32, P]),
+    "qs_compose_tables": (I32, [P, P, P, P, I32, P, P]),
     "qs_fused_measure_work_bytes": (I64, [I64, I32, I32, I32]),
     "qs_fused_measure_finish": (I32, [P, P, I64, I32, I32, I32, P, P]),
     "qs_adjoint_work_bytes": (I64, [I64, I32, I32, I32, I32]),
-    "qs_adjoint_pass": (I32, [P, P, I64, I32, I32, I32, P, P, I32, P, P, I32, P, P]),
+    "qs_adjoint_pass": (I32, [P, P, I64, I32, I32, I32, P, P, I32, P, P, P, P, I32, P, P]),
     "qs_adjoint_finish": (I32, [P, P, P, I32, I64, I32, I32, I32, P]),
     "qs_factor_entries": (I32, [P, P, P, I64, P]),
     "qs_pass_desc_bytes": (I64, []),
-    "qs_pass_desc": (I32, [P, P, P, P, I64, I32, I32, I32, P, P, I32, P, P, P, I32]),
+    "qs_pass_desc": (I32, [P, P, P, P, I64, I32, I32, I32, P, P, I32, P, P, P, P, I32]),
     "qs_items_pass": (I32, [P, P, I32, I32, I32, I32, P]),
     "qs_measure_desc_bytes": (I64, []),
     "qs_measure_desc": (I32, [P, P, P, I64, I32, I32, I32, P]),
     "qs_items_measure_finish": (I32, [P, I32, I32, P]),
     "qs_adjoint_desc_bytes": (I64, []),
-    "qs_adjoint_desc": (I32, [P, P, P, P, I64, I32, I32, I32, P, P, I32, P, P, I32, P]),
+    "qs_adjoint_desc": (I32, [P, P, P, P, I64, I32, I32, I32, P, P, I32, P, P, P, P, I32, P]),
     "qs_items_adjoint": (I32, [P, P, I32, I32, I32, I32, P]),
     "qs_adjoint_finish_desc_bytes": (I64, []),
     "qs_adjoint_finish_desc": (I32, [P, P, P, P, I32, I64, I32, I32, I32]),

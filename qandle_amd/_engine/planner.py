@@ -174,7 +174,7 @@ def plan(n: int, gates: typing.List[Gate], R: int, TB: int, cm: CostModel = None
         L = list(range(K))
         subs = _pack(gates, pending, list(range(n)), R, cm) if pending else [SubPass(list(range(R)), [])]
         return Plan(n, R, TB, [Pass(L, subs, pending)])
-    low = list(range(min(cm.min_low, K)))
+    low = list(range(min(cm.min_low, K - 1)))  # always leave room for a needed position
     while True:
         ranked = [p for p in _rank_positions(gates, pending, set(range(n))) if p not in low]
         L = list(low) + ranked[: K - len(low)]
@@ -206,8 +206,9 @@ def plan(n: int, gates: typing.List[Gate], R: int, TB: int, cm: CostModel = None
 
 
 def swz(l: int) -> int:
-    f = (l >> 5) ^ (l >> 9) ^ (l >> 13)
-    return l ^ ((f & 15) << 1)
+    """LDS slot of tile element l (same XOR-linear map as the kernels' swz)."""
+    f = (l >> 5) ^ (l >> 10) ^ (l >> 15)
+    return l ^ (f & 31)
 
 
 def encode_pass(p: Pass, gates, R: int, TB: int, coef_index, per_sample,
@@ -273,3 +274,166 @@ def plan_stats(pl: Plan, gates, cm: CostModel = None):
         cost = sum(_gate_cost(gates[i], gates[i].kind != RZ or gates[i].pos in Lset, cm) for i in p.gates)
         out.append({"gates": len(p.gates), "subpasses": len(p.subs), "cost": cost + cm.subpass * len(p.subs)})
     return out
+
+
+# ----------------------------------------------------------------------------
+# v2 op format: per sub-pass, phase entries + segments of fused per-bit unitaries
+# ----------------------------------------------------------------------------
+#
+# Single-qubit gates on different bits commute, and every RZ on a bit that is
+# not a register bit commutes with everything else in the sub-pass (the only
+# other use of such a bit is as a CNOT control).  So a sub-pass's gate list is
+# rewritten as:
+#   phases   : RZ gates on non-register bits -> one per-thread phase
+#   segments : [per register bit, the product of its consecutive rotations]
+#              followed by the CNOTs of that stretch, in order
+# A rotation joins the current segment's unitary on its bit unless a CNOT of
+# the segment already touched that bit (then a new segment starts).
+
+SEG_NOUNAPPLY = 2
+SEG_PER_SAMPLE = 1
+
+
+@dataclasses.dataclass
+class Segment:
+    bits: dict                 # register index -> [gate indices, in application order]
+    cnots: list                # gate indices
+
+
+def segment_subpass(sp: SubPass, gates):
+    reg = {q: r for r, q in enumerate(sp.P)}
+    phases, segs = [], []
+    cur = Segment({}, [])
+    touched = set()
+    for gi in sp.gates:
+        g = gates[gi]
+        if g.kind == CNOT:
+            cur.cnots.append(gi)
+            touched.add(reg[g.pos])
+            if g.ctrl in reg:
+                touched.add(reg[g.ctrl])
+            continue
+        if g.pos not in reg:  # RZ (or a Z gradient probe) on a non-register bit
+            phases.append(gi)
+            continue
+        r = reg[g.pos]
+        if r in touched:
+            segs.append(cur)
+            cur = Segment({}, [])
+            touched = set()
+        cur.bits.setdefault(r, []).append(gi)
+    if cur.bits or cur.cnots or not segs:
+        segs.append(cur)
+    return phases, segs
+
+
+class TableBuilder:
+    """Allocates fused-unitary (U) and gradient-generator (Q) entries and records
+    the device recipes that compose them (``qs_compose_tables``).
+
+    Recipe words per U entry: u_index, span (B for a per-sample entry, else
+    1), nconst, q_index, then per constituent (kind, cs_index, cs_per_sample);
+    kind 0/1/2 = RX/RY/RZ, PGRAD + p = a gradient probe with Pauli p (identity
+    rotation)."""
+
+    def __init__(self, B: int):
+        self.B = B
+        self.nu = 0
+        self.nq = 0
+        self.words: list = []
+        self.offsets: list = []
+
+    def add(self, consts, B: int = None):
+        ps = any(c[2] for c in consts)
+        span = (self.B if B is None else B) if ps else 1
+        u, q = self.nu, self.nq
+        self.nu += span
+        self.nq += span * len(consts)
+        self.offsets.append(len(self.words))
+        self.words.extend([u, span, len(consts), q])
+        for c in consts:
+            self.words.extend([int(c[0]), int(c[1]), 1 if c[2] else 0])
+        return u, ps, q
+
+    @property
+    def count(self) -> int:
+        return len(self.offsets)
+
+    def recipes(self) -> np.ndarray:
+        return np.asarray(self.words if self.words else [0], dtype=np.int32)
+
+
+def encode_pass_v2(p: Pass, gates, R: int, TB: int, coef_index, per_sample, tables: TableBuilder,
+                   reverse: bool = False, row_of=None, B: int = None) -> np.ndarray:
+    """int32 words of one pass in the layout of qs_program.h.
+
+    Forward segments list their per-bit unitaries, then their CNOTs (the
+    order they apply in).  ``reverse`` encodes for the adjoint kernel:
+    sub-passes, segments and CNOTs in reverse order, each segment's CNOT
+    words before its unitary entries (the order they are undone in), and
+    every rotation's gradient row ``row_of(gate)`` after its entry."""
+    K = R + TB
+    L = p.L
+    loc = {pos: j for j, pos in enumerate(L)}
+    rows = (lambda gi: row_of(gi)) if (reverse and row_of is not None) else (lambda gi: -1)
+    words = []
+    subs = list(reversed(p.subs)) if reverse else p.subs
+    for sp in subs:
+        Pl = [loc[q] for q in sp.P]
+        G = [j for j in range(K) if j not in Pl]
+        phases, segs = segment_subpass(sp, gates)
+        if reverse:
+            segs = list(reversed(segs))
+        hdr = [0] * HDR
+        hdr[0] = len(segs)
+        hdr[6] = len(phases)
+        for r, j in enumerate(Pl):
+            hdr[HDR_P + r] = j
+            hdr[HDR_SWZ + r] = swz(1 << j)
+            hdr[HDR_GP + r] = L[j]
+        for i, j in enumerate(G):
+            hdr[HDR_G + i] = j
+            hdr[HDR_GG + i] = L[j]
+        words.extend(hdr)
+        reg = {q: r for r, q in enumerate(sp.P)}
+        for gi in phases:
+            g = gates[gi]
+            if g.kind >= PGRAD:  # gradient probe only (Z on a non-register bit)
+                words.extend([g.pos, 0, 2, rows(gi)])
+            else:
+                words.extend([g.pos, int(coef_index(g.slot)), 1 if per_sample(g.slot) else 0, rows(gi)])
+        for seg in segs:
+            cn = list(reversed(seg.cnots)) if reverse else seg.cnots
+            umask = 0
+            for r in seg.bits:
+                umask |= 1 << r
+            words.append(umask | (len(cn) << 8))
+            if reverse:  # the adjoint undoes the segment's CNOTs before its unitaries
+                _cnot_words(words, cn, gates, reg)
+            for r in sorted(seg.bits):
+                gl = seg.bits[r]
+                consts = []
+                probe = False
+                for gi in gl:
+                    g = gates[gi]
+                    if g.kind >= PGRAD:
+                        probe = True
+                        consts.append((g.kind, 0, False))
+                    else:
+                        consts.append((g.kind, int(coef_index(g.slot)), bool(per_sample(g.slot))))
+                u, ps, q = tables.add(consts, B)
+                flags = (SEG_PER_SAMPLE if ps else 0) | (SEG_NOUNAPPLY if probe else 0)
+                words.extend([u, flags, len(gl), q])
+                words.extend(rows(gi) for gi in gl)
+            if not reverse:
+                _cnot_words(words, cn, gates, reg)
+    return np.asarray(words, dtype=np.int32)
+
+
+def _cnot_words(words, cn, gates, reg):
+    for gi in cn:
+        g = gates[gi]
+        if g.ctrl in reg:
+            words.extend([OP_CNOT, reg[g.pos], reg[g.ctrl], 0])
+        else:
+            words.extend([OP_CNOT_UNIFORM, reg[g.pos], g.ctrl, 0])

@@ -180,14 +180,18 @@ class UnevenBatch:
         mbytes = lib.qs_measure_desc_bytes()
         chunks, where = [], []
         pos = 0
+        tables = PL.TableBuilder(1)
         for c in self.circs:
             for p in c.plan.passes:
-                w = PL.encode_pass(p, c.prog.gates, R, TB, c.prog.slot_index.__getitem__,
-                                   lambda s: False)
+                w = PL.encode_pass_v2(p, c.prog.gates, R, TB, lambda s, c=c: c.phi_off + c.prog.slot_index[s],
+                                      lambda s: False, tables)
                 chunks.append(w)
                 where.append(pos)
                 pos += len(w)
         self.fw_words = self._upload_words(chunks)
+        self.fw_chunks = chunks
+        self.fw_tables = fused.upload_tables([], tables, self.device)
+        self.fw_utab = torch.empty(max(tables.nu, 1) * 8, dtype=torch.float32, device=self.device)
         descs = bytearray()
         levels: typing.Dict[int, list] = {}
         k = 0
@@ -202,7 +206,7 @@ class UnevenBatch:
                     buf, L.ctypes.byref(ntiles_c), st_ptr, None if pi == 0 else st_ptr, c.B, c.n, R, TB,
                     lpos.ctypes.data_as(L.ctypes.POINTER(L.ctypes.c_int32)),
                     self._addr(self.fw_words, where[k]), len(p.subs),
-                    self._addr(self.cs, 2 * c.phi_off), self._addr(self.efac, 4 * c.efac_off),
+                    L.ptr(self.cs), L.ptr(self.fw_utab), self._addr(self.efac, 4 * c.efac_off),
                     self._addr(self.mpart, c.mpart_off) if last else None, 1,
                 )
                 L.check(rc, f"qs_pass_desc(circuit {ci}, pass {pi})")
@@ -235,6 +239,7 @@ class UnevenBatch:
         self.gtot = torch.empty(max(self.totals["gtot"], 1), dtype=torch.float32, device=dev)
         chunks, steps = [], []
         pos = 0
+        tables = PL.TableBuilder(1)
         for c in self.circs:
             rows = {}
             for gi, g in enumerate(c.prog.gates):
@@ -242,13 +247,13 @@ class UnevenBatch:
                     rows[gi] = len(rows)
             seq = []
             for p in reversed(c.plan.passes):
-                w = PL.encode_pass(p, c.prog.gates, R, TB, c.prog.slot_index.__getitem__,
-                                   lambda s: False, reverse=True, row_of=rows.__getitem__)
+                w = PL.encode_pass_v2(p, c.prog.gates, R, TB, lambda s, c=c: c.phi_off + c.prog.slot_index[s],
+                                      lambda s: False, tables, reverse=True, row_of=rows.__getitem__)
                 seq.append((p, w))
             if c.gplan is not None:
                 for p in c.gplan.passes:
-                    w = PL.encode_pass(p, c.ggates, R, TB, lambda s: 0, lambda s: False, reverse=True,
-                                       row_of=lambda gi, c=c: c.nrot + gi)
+                    w = PL.encode_pass_v2(p, c.ggates, R, TB, lambda s: 0, lambda s: False, tables,
+                                          reverse=True, row_of=lambda gi, c=c: c.nrot + gi)
                     seq.append((p, w))
             st = []
             for p, w in seq:
@@ -257,6 +262,11 @@ class UnevenBatch:
                 pos += len(w)
             steps.append(st)
         self.bw_words = self._upload_words(chunks)
+        self.bw_chunks = chunks
+        self.bw_steps = steps
+        self.bw_tables = fused.upload_tables([], tables, dev)
+        self.bw_utab = torch.empty(max(tables.nu, 1) * 8, dtype=torch.float32, device=dev)
+        self.bw_qtab = torch.empty(max(tables.nq, 1) * 8, dtype=torch.float32, device=dev)
         dbytes = lib.qs_adjoint_desc_bytes()
         buf = (L.ctypes.c_char * dbytes)()
         ntiles_c = L.ctypes.c_int64(0)
@@ -269,8 +279,8 @@ class UnevenBatch:
                 rc = lib.qs_adjoint_desc(
                     buf, L.ctypes.byref(ntiles_c), self._addr(self.psi, c.state_off), self._addr(self.lam, c.state_off),
                     c.B, c.n, R, TB, lpos.ctypes.data_as(L.ctypes.POINTER(L.ctypes.c_int32)),
-                    self._addr(self.bw_words, wpos), nsub, self._addr(self.cs, 2 * c.phi_off),
-                    self._addr(self.gpart, c.gpart_off), c.rows,
+                    self._addr(self.bw_words, wpos), nsub, L.ptr(self.cs), L.ptr(self.bw_utab),
+                    L.ptr(self.bw_qtab), self._addr(self.gpart, c.gpart_off), c.rows,
                     self._addr(self.w, c.out_off) if li == 0 else None,
                 )
                 L.check(rc, f"qs_adjoint_desc(circuit {ci}, step {li})")
@@ -319,11 +329,17 @@ class UnevenBatch:
     def _stream(self):
         return torch.cuda.current_stream(self.device).cuda_stream
 
+    def _compose(self, ws, utab, qtab, st):
+        L.check(self.lib.qs_compose_tables(L.ptr(utab), None if qtab is None else L.ptr(qtab),
+                                           L.ptr(ws.recipes), L.ptr(ws.offsets),
+                                           ws.nrec, L.ptr(self.cs), st), "qs_compose_tables")
+
     def run_forward(self):
         lib, st = self.lib, self._stream()
         L.check(lib.qs_half_angle_coeffs(L.ptr(self.cs), L.ptr(self.phi), self.phi.numel(), st), "coeffs")
         L.check(lib.qs_factor_entries(L.ptr(self.efac), L.ptr(self.x), L.ptr(self.entries), self.nentries, st),
                 "qs_factor_entries")
+        self._compose(self.fw_tables, self.fw_utab, None, st)
         for items, cnt in zip(self.fw_levels, self.fw_level_items):
             L.check(lib.qs_items_pass(L.ptr(self.fw_descs), L.ptr(items), cnt, self.R, self.TB, self.grid, st),
                     "qs_items_pass")
@@ -336,6 +352,7 @@ class UnevenBatch:
             self._build_backward()
         lib, st = self.lib, self._stream()
         self.psi.copy_(self.state)
+        self._compose(self.bw_tables, self.bw_utab, self.bw_qtab, st)
         for items, cnt in zip(self.bw_levels, self.bw_level_items):
             L.check(lib.qs_items_adjoint(L.ptr(self.bw_descs), L.ptr(items), cnt, self.R, self.TB, self.grid, st),
                     "qs_items_adjoint")

@@ -24,6 +24,8 @@ struct AdjArgs {
   float *psi;
   float *lam;
   const float *cs;
+  const float *utab;  // fused per-bit unitaries (un-applied as U^dagger)
+  const float *qtab;  // gradient generators of their constituents
   float *gpart;
   const int32_t *words;
   const float *seed_w;  // non-null: lam = 2 psi sum_q w[b, q] [bit n-1-q = 0], (B, n) row-major
@@ -110,9 +112,13 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &A, const int64_t t, 
         b = reinterpret_cast<const float4 *>(A.lam)[gi >> 1];
       }
     }
-    const int l = swz(2 * (it * T + tid));
-    *reinterpret_cast<float4 *>(&tp[l]) = a;
-    *reinterpret_cast<float4 *>(&tl[l]) = b;
+    const int sl = swz(2 * (it * T + tid));
+    if (sl & 1) {
+      a = make_float4(a.z, a.w, a.x, a.y);
+      b = make_float4(b.z, b.w, b.x, b.y);
+    }
+    *reinterpret_cast<float4 *>(&tp[sl & ~1]) = a;
+    *reinterpret_cast<float4 *>(&tl[sl & ~1]) = b;
   }
   __syncthreads();
 
@@ -120,7 +126,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &A, const int64_t t, 
   const int32_t *w = A.words;
   const float2 *cs = reinterpret_cast<const float2 *>(A.cs);
   for (int s = 0; s < A.nsub; ++s) {
-    const int nops = w[0];
+    const int nseg = w[0], nphase = w[6];
     int gl = 0;
     int64_t gbase = base;
 #pragma unroll
@@ -149,74 +155,56 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &A, const int64_t t, 
       vl[j] = tl[addr(j)];
     }
 
-    const int32_t *op = w + kHdr;
-    for (int o = 0; o < nops; ++o, op += 4) {
-      const int code = op[0];
-      const int a = op[1], b = op[2];
-      const int kind = code & 0xff;
-      if (kind == QS_OP_CNOT) {
-        cnot_dispatch<R>(vp, b, a);
-        cnot_dispatch<R>(vl, b, a);
-        continue;
-      }
-      if (kind == QS_OP_CNOT_UNIFORM) {
-        if ((gbase >> b) & 1) {
-          swap_dispatch<R>(vp, a);
-          swap_dispatch<R>(vl, a);
-        }
-        continue;
-      }
-      float g;
-      if (kind == QS_OP_PAULI_GRAD) {
-        const int pauli = op[3] & 3;
-        if (op[3] & QS_PAULI_UNIFORM) {  // Z on a bit that is constant per thread
-          float acc = 0.f;
-#pragma unroll
-          for (int j = 0; j < NR; ++j) acc += vl[j].x * vp[j].y - vl[j].y * vp[j].x;
-          g = ((gbase >> a) & 1) ? -acc : acc;
-        } else if (pauli == QS_RX) {
-          g = im_dispatch<QS_RX, R>(vl, vp, a);
-        } else if (pauli == QS_RY) {
-          g = im_dispatch<QS_RY, R>(vl, vp, a);
-        } else {
-          g = im_dispatch<QS_RZ, R>(vl, vp, a);
-        }
-        for (int off = 1; off < (1 << wlog); off <<= 1) g += __shfl_xor(g, off, 64);
-        if (leader) A.gpart[(int64_t)b * A.NS + u] = g;
-        continue;
-      }
-      const int64_t idx = op[3] + ((code & QS_OP_PER_SAMPLE) ? csample : 0);
-      const float2 c_s = cs[idx];
-      if (kind == QS_OP_RX) {
-        g = im_dispatch<QS_RX, R>(vl, vp, a);
-        rot_dispatch<QS_RX, R>(vp, a, c_s.x, -c_s.y);
-        rot_dispatch<QS_RX, R>(vl, a, c_s.x, -c_s.y);
-      } else if (kind == QS_OP_RY) {
-        g = im_dispatch<QS_RY, R>(vl, vp, a);
-        rot_dispatch<QS_RY, R>(vp, a, c_s.x, -c_s.y);
-        rot_dispatch<QS_RY, R>(vl, a, c_s.x, -c_s.y);
-      } else if (kind == QS_OP_RZ) {
-        g = im_dispatch<QS_RZ, R>(vl, vp, a);
-        rot_dispatch<QS_RZ, R>(vp, a, c_s.x, -c_s.y);
-        rot_dispatch<QS_RZ, R>(vl, a, c_s.x, -c_s.y);
-      } else {  // QS_OP_RZ_UNIFORM on global position a (constant per thread)
-        const bool one = (gbase >> a) & 1;
-        float acc = 0.f;
-#pragma unroll
-        for (int j = 0; j < NR; ++j) acc += vl[j].x * vp[j].y - vl[j].y * vp[j].x;
-        g = one ? -acc : acc;
-        const float c = c_s.x;
-        const float sn = one ? c_s.y : -c_s.y;  // inverse phase
-#pragma unroll
-        for (int j = 0; j < NR; ++j) {
-          float2 x = vp[j];
-          vp[j] = make_float2(c * x.x + sn * x.y, c * x.y - sn * x.x);
-          x = vl[j];
-          vl[j] = make_float2(c * x.x + sn * x.y, c * x.y - sn * x.x);
-        }
-      }
+    auto reduce_put = [&](float g, int row) {
       for (int off = 1; off < (1 << wlog); off <<= 1) g += __shfl_xor(g, off, 64);
-      if (leader) A.gpart[(int64_t)b * A.NS + u] = g;
+      if (leader && row >= 0) A.gpart[(int64_t)row * A.NS + u] = g;
+    };
+    const int32_t *ph = w + kHdr;
+    const int32_t *op = ph + 4 * nphase;
+    if (nphase) {
+      // every phase entry commutes with the whole sub-pass: its gradient is read
+      // at the sub-pass output, where Im<lam|Z_p|psi> = (+/-) sum Im(conj(lam) psi)
+      float S = 0.f;
+#pragma unroll
+      for (int j = 0; j < NR; ++j) S += vl[j].x * vp[j].y - vl[j].y * vp[j].x;
+      for (int k = 0; k < nphase; ++k) reduce_put(((gbase >> ph[4 * k]) & 1) ? -S : S, ph[4 * k + 3]);
+      const float2 z = phase_product(ph, nphase, cs, csample, gbase, true);
+#pragma unroll
+      for (int j = 0; j < NR; ++j) {
+        vp[j] = cmul(z, vp[j]);
+        vl[j] = cmul(z, vl[j]);
+      }
+    }
+    for (int sg = 0; sg < nseg; ++sg) {
+      const int head = *op++;
+      const int umask = head & 0xff, ncnot = head >> 8;
+      for (int c = 0; c < ncnot; ++c, op += 4) {
+        if (op[0] == QS_OP_CNOT) {
+          cnot_dispatch<R>(vp, op[2], op[1]);
+          cnot_dispatch<R>(vl, op[2], op[1]);
+        } else if ((gbase >> op[2]) & 1) {
+          swap_dispatch<R>(vp, op[1]);
+          swap_dispatch<R>(vl, op[1]);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if ((umask >> r) & 1) {
+          const int flags = op[1], nconst = op[2];
+          const int64_t var = (flags & QS_SEG_PER_SAMPLE) ? csample : 0;
+          const int64_t span = (flags & QS_SEG_PER_SAMPLE) ? A.B : 1;
+          float m[8];
+          cross_density_at<R>(vp, vl, r, m);
+          for (int k = 0; k < nconst; ++k)
+            reduce_put(im_trace(A.qtab + 8 * (op[3] + k * span + var), m), op[4 + k]);
+          if (!(flags & QS_SEG_NOUNAPPLY)) {
+            const float *uu = A.utab + 8 * (op[0] + var);
+            apply_udag_at<R>(vp, r, uu);
+            apply_udag_at<R>(vl, r, uu);
+          }
+          op += 4 + nconst;
+        }
+      }
     }
 
 #pragma unroll
@@ -225,7 +213,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &A, const int64_t t, 
       tl[addr(j)] = vl[j];
     }
     __syncthreads();
-    w += kHdr + 4 * nops;
+    w = op;
   }
 
 #pragma unroll
@@ -235,10 +223,16 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &A, const int64_t t, 
     for (int i = 0; i < R - 1; ++i)
       if ((it >> i) & 1) g_it |= int64_t(1) << A.lpos[TB + 1 + i];
     const int64_t gi = base | g_tid | g_it;
-    const int l = swz(2 * (it * T + tid));
+    const int sl = swz(2 * (it * T + tid));
+    float4 a = *reinterpret_cast<const float4 *>(&tp[sl & ~1]);
+    float4 b = *reinterpret_cast<const float4 *>(&tl[sl & ~1]);
+    if (sl & 1) {
+      a = make_float4(a.z, a.w, a.x, a.y);
+      b = make_float4(b.z, b.w, b.x, b.y);
+    }
     if (gi < total) {
-      reinterpret_cast<float4 *>(A.psi)[gi >> 1] = *reinterpret_cast<const float4 *>(&tp[l]);
-      reinterpret_cast<float4 *>(A.lam)[gi >> 1] = *reinterpret_cast<const float4 *>(&tl[l]);
+      reinterpret_cast<float4 *>(A.psi)[gi >> 1] = a;
+      reinterpret_cast<float4 *>(A.lam)[gi >> 1] = b;
     }
   }
 }
@@ -343,7 +337,8 @@ int64_t qs_adjoint_work_bytes(int64_t B, int n, int R, int TB, int rows) {
 
 static int fill_adj_args(AdjArgs &A, int64_t *ntiles, float *psi, float *lam, int64_t B, int n, int R,
                          int TB, const int32_t *lpos, const int32_t *words, int nsub, const float *cs,
-                         float *gpart, int rows, const float *seed_w) {
+                         const float *utab, const float *qtab, float *gpart, int rows,
+                         const float *seed_w) {
   if (B < 1 || n < 1 || n > QS_MAX_QUBITS || !psi || !lam || !lpos || !words || nsub < 1 ||
       !gpart || rows < 0 || !qs_fused_supported(R, TB) || R > n)
     return QS_EINVAL;
@@ -351,6 +346,8 @@ static int fill_adj_args(AdjArgs &A, int64_t *ntiles, float *psi, float *lam, in
   A.psi = psi;
   A.lam = lam;
   A.cs = cs;
+  A.utab = utab;
+  A.qtab = qtab;
   A.gpart = gpart;
   A.words = words;
   A.seed_w = seed_w;
@@ -385,11 +382,12 @@ static int fill_adj_args(AdjArgs &A, int64_t *ntiles, float *psi, float *lam, in
 extern "C" {
 
 int qs_adjoint_pass(float *psi, float *lam, int64_t B, int n, int R, int TB, const int32_t *lpos,
-                    const int32_t *words, int nsub, const float *cs, float *gpart, int rows,
-                    const float *seed_w, void *stream) {
+                    const int32_t *words, int nsub, const float *cs, const float *utab,
+                    const float *qtab, float *gpart, int rows, const float *seed_w, void *stream) {
   AdjArgs A;
   int64_t ntiles;
-  int st0 = fill_adj_args(A, &ntiles, psi, lam, B, n, R, TB, lpos, words, nsub, cs, gpart, rows, seed_w);
+  int st0 = fill_adj_args(A, &ntiles, psi, lam, B, n, R, TB, lpos, words, nsub, cs, utab, qtab, gpart,
+                          rows, seed_w);
   if (st0) return st0;
   if (ntiles > 0x7fffffff) return QS_EUNSUPPORTED;
   const bool multi = n < R + TB;
@@ -413,10 +411,11 @@ int64_t qs_adjoint_desc_bytes(void) { return (int64_t)sizeof(AdjArgs); }
 
 int qs_adjoint_desc(void *desc, int64_t *ntiles, float *psi, float *lam, int64_t B, int n, int R,
                     int TB, const int32_t *lpos, const int32_t *words, int nsub, const float *cs,
-                    float *gpart, int rows, const float *seed_w) {
+                    const float *utab, const float *qtab, float *gpart, int rows, const float *seed_w) {
   if (!desc || !ntiles) return QS_EINVAL;
   AdjArgs A;
-  int st0 = fill_adj_args(A, ntiles, psi, lam, B, n, R, TB, lpos, words, nsub, cs, gpart, rows, seed_w);
+  int st0 = fill_adj_args(A, ntiles, psi, lam, B, n, R, TB, lpos, words, nsub, cs, utab, qtab, gpart,
+                          rows, seed_w);
   if (st0) return st0;
   memcpy(desc, &A, sizeof(AdjArgs));
   return QS_OK;

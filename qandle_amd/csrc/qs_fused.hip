@@ -28,6 +28,7 @@ struct PassArgs {
   float *state;
   const float *src;
   const float *cs;
+  const float *utab;  // fused per-bit unitaries, 8 floats each
   const float *efac;
   float *mpart;
   const int32_t *words;
@@ -79,8 +80,9 @@ __device__ __forceinline__ void pass_tile(const PassArgs &A, const int64_t t, fl
       const int64_t gi = base | g_tid | g_it;
       float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
       if (gi < total) val = reinterpret_cast<const float4 *>(A.src)[gi >> 1];
-      const int l = 2 * (it * T + tid);
-      *reinterpret_cast<float4 *>(&tile[swz(l)]) = val;
+      const int sl = swz(2 * (it * T + tid));
+      if (sl & 1) val = make_float4(val.z, val.w, val.x, val.y);
+      *reinterpret_cast<float4 *>(&tile[sl & ~1]) = val;
     }
     __syncthreads();
   }
@@ -88,7 +90,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs &A, const int64_t t, fl
   float2 v[NR];
   const int32_t *w = A.words;
   for (int s = 0; s < A.nsub; ++s) {
-    const int nops = w[0];
+    const int nseg = w[0], nphase = w[6];
     int gl = 0;
     int64_t gbase = base;
 #pragma unroll
@@ -147,7 +149,32 @@ __device__ __forceinline__ void pass_tile(const PassArgs &A, const int64_t t, fl
 
     // a padding sample past B (last tile of a multi-sample pass) holds zeros;
     // it reads sample 0's coefficients so every table access stays in bounds
-    run_ops<R>(v, w + kHdr, nops, reinterpret_cast<const float2 *>(A.cs), live ? sample : 0, gbase);
+    const int64_t csample = live ? sample : 0;
+    const int32_t *ph = w + kHdr;
+    const int32_t *op = ph + 4 * nphase;
+    for (int sg = 0; sg < nseg; ++sg) {
+      const int head = *op++;
+      const int umask = head & 0xff, ncnot = head >> 8;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if ((umask >> r) & 1) {
+          const int64_t ui = op[0] + ((op[1] & QS_SEG_PER_SAMPLE) ? csample : 0);
+          apply_u_at<R>(v, r, A.utab + 8 * ui);
+          op += 4 + op[2];
+        }
+      }
+      for (int c = 0; c < ncnot; ++c, op += 4) {
+        if (op[0] == QS_OP_CNOT)
+          cnot_dispatch<R>(v, op[2], op[1]);
+        else if ((gbase >> op[2]) & 1)
+          swap_dispatch<R>(v, op[1]);
+      }
+    }
+    if (nphase) {
+      const float2 z = phase_product(ph, nphase, reinterpret_cast<const float2 *>(A.cs), csample, gbase, false);
+#pragma unroll
+      for (int j = 0; j < NR; ++j) v[j] = cmul(z, v[j]);
+    }
 
     if (s == A.nsub - 1 && A.mpart) {
       // MeasureProbability partials: for every local bit, sum of |a|^2 over the
@@ -219,7 +246,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs &A, const int64_t t, fl
 #pragma unroll
     for (int j = 0; j < NR; ++j) tile[addr(j)] = v[j];
     __syncthreads();
-    w += kHdr + 4 * nops;
+    w = op;
   }
 
   if (A.store) {
@@ -230,9 +257,10 @@ __device__ __forceinline__ void pass_tile(const PassArgs &A, const int64_t t, fl
       for (int i = 0; i < R - 1; ++i)
         if ((it >> i) & 1) g_it |= int64_t(1) << A.lpos[TB + 1 + i];
       const int64_t gi = base | g_tid | g_it;
-      const int l = 2 * (it * T + tid);
-      if (gi < total)
-        reinterpret_cast<float4 *>(A.state)[gi >> 1] = *reinterpret_cast<const float4 *>(&tile[swz(l)]);
+      const int sl = swz(2 * (it * T + tid));
+      float4 val = *reinterpret_cast<const float4 *>(&tile[sl & ~1]);
+      if (sl & 1) val = make_float4(val.z, val.w, val.x, val.y);
+      if (gi < total) reinterpret_cast<float4 *>(A.state)[gi >> 1] = val;
     }
   }
 }
@@ -376,6 +404,85 @@ __global__ void k_factor_entries(float *efac, const float *x, const int32_t *ent
   }
 }
 
+// ---------------------------------------------------------------------------
+// Table composition: fused per-bit unitaries and their gradient generators.
+// Recipe r (at words + off[r]): u_index, span, nconst, q_index, then per
+// constituent (kind, cs_index, cs_per_sample) in application order.  For every
+// sample variant b < span (the circuit's batch for a per-sample entry, else 1):
+//   utab[u_index + b]        = R_m ... R_2 R_1
+//   qtab[q_index + k*span + b] = V_k P_k V_k^dagger,  V_k = R_m ... R_{k+1}
+// (the generator that turns the cross density at the unitary's output into
+// Im<lam|P_k|psi> at constituent k's output).  kind >= 8 is a gradient probe:
+// identity rotation with Pauli kind - 8.
+// ---------------------------------------------------------------------------
+struct C2 {
+  double re, im;
+};
+__device__ __forceinline__ C2 cm(C2 a, C2 b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ C2 ca(C2 a, C2 b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ C2 cc(C2 a) { return {a.re, -a.im}; }
+struct M2 {
+  C2 a, b, c, d;  // [[a, b], [c, d]]
+};
+__device__ __forceinline__ M2 mm(const M2 &x, const M2 &y) {
+  return {ca(cm(x.a, y.a), cm(x.b, y.c)), ca(cm(x.a, y.b), cm(x.b, y.d)),
+          ca(cm(x.c, y.a), cm(x.d, y.c)), ca(cm(x.c, y.b), cm(x.d, y.d))};
+}
+__device__ __forceinline__ M2 mdag(const M2 &x) { return {cc(x.a), cc(x.c), cc(x.b), cc(x.d)}; }
+
+__device__ __forceinline__ M2 rot_matrix(int kind, float c, float s) {
+  const C2 z = {0, 0};
+  if (kind == QS_RX) return {{c, 0}, {0, -s}, {0, -s}, {c, 0}};
+  if (kind == QS_RY) return {{c, 0}, {-s, 0}, {s, 0}, {c, 0}};
+  if (kind == QS_RZ) return {{c, -s}, z, z, {c, s}};
+  return {{1, 0}, z, z, {1, 0}};  // gradient probe
+}
+__device__ __forceinline__ M2 pauli(int p) {
+  const C2 z = {0, 0};
+  if (p == QS_RX) return {z, {1, 0}, {1, 0}, z};
+  if (p == QS_RY) return {z, {0, -1}, {0, 1}, z};
+  return {{1, 0}, z, z, {-1, 0}};
+}
+__device__ __forceinline__ void store_m2(float *o, const M2 &x) {
+  o[0] = (float)x.a.re; o[1] = (float)x.a.im; o[2] = (float)x.b.re; o[3] = (float)x.b.im;
+  o[4] = (float)x.c.re; o[5] = (float)x.c.im; o[6] = (float)x.d.re; o[7] = (float)x.d.im;
+}
+
+__device__ __forceinline__ M2 recipe_rot(const int32_t *w, int k, int64_t b, const float2 *csv, int *p) {
+  const int kind = w[4 + 3 * k], ci = w[5 + 3 * k], cps = w[6 + 3 * k];
+  *p = kind < 8 ? kind : kind - 8;
+  if (kind >= 8) return rot_matrix(-1, 1.f, 0.f);
+  const float2 v = csv[ci + (cps ? b : 0)];
+  return rot_matrix(kind, v.x, v.y);
+}
+
+__global__ void __launch_bounds__(64)
+k_compose(float *utab, float *qtab, const int32_t *words, const int32_t *off, int nrec,
+          const float *cs) {
+  const float2 *csv = reinterpret_cast<const float2 *>(cs);
+  for (int r = blockIdx.x; r < nrec; r += gridDim.x) {
+    const int32_t *w = words + off[r];
+    const int64_t uidx = w[0];
+    const int64_t span = w[1];
+    const int m = w[2];
+    const int64_t qidx = w[3];
+    for (int64_t b = threadIdx.x; b < span; b += blockDim.x) {
+      int p;
+      M2 U = {{1, 0}, {0, 0}, {0, 0}, {1, 0}};
+      for (int k = 0; k < m; ++k) U = mm(recipe_rot(w, k, b, csv, &p), U);
+      store_m2(utab + 8 * (uidx + b), U);
+      if (!qtab) continue;
+      M2 V = {{1, 0}, {0, 0}, {0, 0}, {1, 0}};
+      for (int k = m - 1; k >= 0; --k) {
+        const M2 Rk = recipe_rot(w, k, b, csv, &p);
+        const M2 Q = mm(mm(V, pauli(p)), mdag(V));
+        store_m2(qtab + 8 * (qidx + k * span + b), Q);
+        V = mm(V, Rk);
+      }
+    }
+  }
+}
+
 template <int R, int TB>
 static int launch_pass(const PassArgs &A, int64_t ntiles, hipStream_t st) {
   if (A.multi)
@@ -414,7 +521,7 @@ int64_t qs_fused_measure_work_bytes(int64_t B, int n, int R, int TB) {
 // the work-item descriptors).  Returns QS_OK or QS_EINVAL.
 static int fill_pass_args(PassArgs &A, float *state, const float *src, int64_t B, int n, int R, int TB,
                           const int32_t *lpos, const int32_t *words, int nsub, const float *cs,
-                          const float *efac, float *mpart, int store) {
+                          const float *utab, const float *efac, float *mpart, int store) {
   if (B < 1 || n < 1 || n > QS_MAX_QUBITS || !state || !lpos || !words || nsub < 1 ||
       !qs_fused_supported(R, TB) || R > n || (!src && !efac))
     return QS_EINVAL;
@@ -422,6 +529,7 @@ static int fill_pass_args(PassArgs &A, float *state, const float *src, int64_t B
   A.state = state;
   A.src = src;
   A.cs = cs;
+  A.utab = utab;
   A.efac = efac;
   A.mpart = mpart;
   A.words = words;
@@ -456,9 +564,9 @@ extern "C" {
 
 int qs_fused_pass(float *state, const float *src, int64_t B, int n, int R, int TB,
                   const int32_t *lpos, const int32_t *words, int nsub, const float *cs,
-                  const float *efac, float *mpart, int store, void *stream) {
+                  const float *utab, const float *efac, float *mpart, int store, void *stream) {
   PassArgs A;
-  int st0 = fill_pass_args(A, state, src, B, n, R, TB, lpos, words, nsub, cs, efac, mpart, store);
+  int st0 = fill_pass_args(A, state, src, B, n, R, TB, lpos, words, nsub, cs, utab, efac, mpart, store);
   if (st0) return st0;
   const int K = R + TB;
   const int64_t ntiles = fused_tiles(B, n, K);
@@ -479,6 +587,17 @@ int qs_fused_pass(float *state, const float *src, int64_t B, int n, int R, int T
   return QS_EUNSUPPORTED;
 }
 
+int qs_compose_tables(float *utab, float *qtab, const int32_t *recipes, const int32_t *offsets,
+                      int nrec, const float *cs, void *stream) {
+  if (nrec < 0 || (nrec > 0 && (!utab || !recipes || !offsets || !cs)))
+    return QS_EINVAL;
+  if (nrec == 0) return QS_OK;
+  const unsigned g = (unsigned)(nrec < 65536 ? nrec : 65536);
+  hipLaunchKernelGGL(k_compose, dim3(g), dim3(64), 0, (hipStream_t)stream, utab, qtab, recipes, offsets,
+                     nrec, cs);
+  return launch_status();
+}
+
 int qs_factor_entries(float *efac, const float *x, const int32_t *entries, int64_t count,
                       void *stream) {
   if (count < 0 || (count > 0 && (!efac || !entries))) return QS_EINVAL;
@@ -494,10 +613,10 @@ int64_t qs_pass_desc_bytes(void) { return (int64_t)sizeof(PassArgs); }
 
 int qs_pass_desc(void *desc, int64_t *ntiles, float *state, const float *src, int64_t B, int n,
                  int R, int TB, const int32_t *lpos, const int32_t *words, int nsub, const float *cs,
-                 const float *efac, float *mpart, int store) {
+                 const float *utab, const float *efac, float *mpart, int store) {
   if (!desc || !ntiles) return QS_EINVAL;
   PassArgs A;
-  int st0 = fill_pass_args(A, state, src, B, n, R, TB, lpos, words, nsub, cs, efac, mpart, store);
+  int st0 = fill_pass_args(A, state, src, B, n, R, TB, lpos, words, nsub, cs, utab, efac, mpart, store);
   if (st0) return st0;
   *ntiles = fused_tiles(B, n, R + TB);
   memcpy(desc, &A, sizeof(PassArgs));

@@ -17,12 +17,126 @@ constexpr int kHdrG = 24;     // [24, 24+TB): thread-bit local bits
 constexpr int kHdrGG = 40;    // [40, 40+TB): global position of each thread bit
 
 
-// XOR swizzle of an LDS amplitude index: folds bits >= 5 into bits 1..4 so
-// that lanes that differ only in high local bits land in different banks.
-// Linear over XOR and keeps bit 0 (amplitude pairs stay 16-byte vectors).
+// XOR swizzle of an LDS amplitude index: folds the bits >= 5 into bits 0..4
+// so that lanes that differ only in high local bits land in different banks.
+// Linear over XOR (so register addresses are base ^ stride chains) and a
+// bijection (bits >= 5 are unchanged).  An amplitude pair (2m, 2m+1) stays in
+// one 16-byte slot, in swapped order when bit 0 of swz(2m) is set.
 __host__ __device__ __forceinline__ int swz(int l) {
-  int f = (l >> 5) ^ (l >> 9) ^ (l >> 13);
-  return l ^ ((f & 15) << 1);
+  int f = (l >> 5) ^ (l >> 10) ^ (l >> 15);
+  return l ^ (f & 31);
+}
+
+// ---------------------------------------------------------------------------
+// v2 per-bit operations.  A 2x2 complex matrix is 8 floats
+// (u00.re, u00.im, u01.re, u01.im, u10.re, u10.im, u11.re, u11.im), column
+// convention: (a0, a1) <- (u00 a0 + u01 a1, u10 a0 + u11 a1).
+// ---------------------------------------------------------------------------
+template <int BIT, int NR>
+__device__ __forceinline__ void apply_u(float2 (&v)[NR], const float *u) {
+  const float a = u[0], b = u[1], c = u[2], d = u[3], e = u[4], f = u[5], g = u[6], h = u[7];
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+    if (!(j & (1 << BIT))) {
+      const float2 x = v[j], y = v[j | (1 << BIT)];
+      v[j] = make_float2(a * x.x - b * x.y + c * y.x - d * y.y, a * x.y + b * x.x + c * y.y + d * y.x);
+      v[j | (1 << BIT)] = make_float2(e * x.x - f * x.y + g * y.x - h * y.y, e * x.y + f * x.x + g * y.y + h * y.x);
+    }
+  }
+}
+
+// (a0, a1) <- U^dagger (a0, a1) = (conj(u00) a0 + conj(u10) a1, conj(u01) a0 + conj(u11) a1)
+template <int BIT, int NR>
+__device__ __forceinline__ void apply_udag(float2 (&v)[NR], const float *u) {
+  const float a = u[0], b = -u[1], c = u[4], d = -u[5], e = u[2], f = -u[3], g = u[6], h = -u[7];
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+    if (!(j & (1 << BIT))) {
+      const float2 x = v[j], y = v[j | (1 << BIT)];
+      v[j] = make_float2(a * x.x - b * x.y + c * y.x - d * y.y, a * x.y + b * x.x + c * y.y + d * y.x);
+      v[j | (1 << BIT)] = make_float2(e * x.x - f * x.y + g * y.x - h * y.y, e * x.y + f * x.x + g * y.y + h * y.x);
+    }
+  }
+}
+
+// Cross density of bit BIT over the thread's registers: M_ab = sum psi_a conj(lam_b),
+// 8 floats (m00.re, m00.im, m01.re, m01.im, m10.re, m10.im, m11.re, m11.im).
+template <int BIT, int NR>
+__device__ __forceinline__ void cross_density(const float2 (&p)[NR], const float2 (&l)[NR], float *m) {
+  float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f, m6 = 0.f, m7 = 0.f;
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+    if (!(j & (1 << BIT))) {
+      const float2 p0 = p[j], p1 = p[j | (1 << BIT)], l0 = l[j], l1 = l[j | (1 << BIT)];
+      // psi_a * conj(lam_b) = (pa.x lb.x + pa.y lb.y) + i (pa.y lb.x - pa.x lb.y)
+      m0 += p0.x * l0.x + p0.y * l0.y;  m1 += p0.y * l0.x - p0.x * l0.y;
+      m2 += p0.x * l1.x + p0.y * l1.y;  m3 += p0.y * l1.x - p0.x * l1.y;
+      m4 += p1.x * l0.x + p1.y * l0.y;  m5 += p1.y * l0.x - p1.x * l0.y;
+      m6 += p1.x * l1.x + p1.y * l1.y;  m7 += p1.y * l1.x - p1.x * l1.y;
+    }
+  }
+  m[0] = m0; m[1] = m1; m[2] = m2; m[3] = m3; m[4] = m4; m[5] = m5; m[6] = m6; m[7] = m7;
+}
+
+// run-time bit -> compile-time template (the switch folds away once the
+// caller's loop over register bits is unrolled)
+template <int R>
+__device__ __forceinline__ void apply_u_at(float2 (&v)[1 << R], int r, const float *u) {
+  switch (r) {
+    case 0: apply_u<0>(v, u); break;
+    case 1: if constexpr (R > 1) apply_u<1>(v, u); break;
+    case 2: if constexpr (R > 2) apply_u<2>(v, u); break;
+    case 3: if constexpr (R > 3) apply_u<3>(v, u); break;
+    case 4: if constexpr (R > 4) apply_u<4>(v, u); break;
+    default: break;
+  }
+}
+template <int R>
+__device__ __forceinline__ void apply_udag_at(float2 (&v)[1 << R], int r, const float *u) {
+  switch (r) {
+    case 0: apply_udag<0>(v, u); break;
+    case 1: if constexpr (R > 1) apply_udag<1>(v, u); break;
+    case 2: if constexpr (R > 2) apply_udag<2>(v, u); break;
+    case 3: if constexpr (R > 3) apply_udag<3>(v, u); break;
+    case 4: if constexpr (R > 4) apply_udag<4>(v, u); break;
+    default: break;
+  }
+}
+template <int R>
+__device__ __forceinline__ void cross_density_at(const float2 (&p)[1 << R], const float2 (&l)[1 << R], int r, float *m) {
+  switch (r) {
+    case 0: cross_density<0>(p, l, m); break;
+    case 1: if constexpr (R > 1) cross_density<1>(p, l, m); break;
+    case 2: if constexpr (R > 2) cross_density<2>(p, l, m); break;
+    case 3: if constexpr (R > 3) cross_density<3>(p, l, m); break;
+    case 4: if constexpr (R > 4) cross_density<4>(p, l, m); break;
+    default: break;
+  }
+}
+
+// Per-thread phase of a sub-pass's RZ gates on non-register bits:
+// prod_k e^{-/+ i phi_k/2} by the thread's value of bit pos_k.
+__device__ __forceinline__ float2 phase_product(const int32_t *ph, int nphase, const float2 *cs, int64_t csample,
+                                                int64_t gbase, bool inverse) {
+  float pr = 1.f, pi = 0.f;
+  for (int k = 0; k < nphase; ++k) {
+    const int fl = ph[4 * k + 2];
+    if (fl & QS_SEG_NOUNAPPLY) continue;
+    const float2 c_s = cs[ph[4 * k + 1] + ((fl & QS_SEG_PER_SAMPLE) ? csample : 0)];
+    float sn = ((gbase >> ph[4 * k]) & 1) ? -c_s.y : c_s.y;
+    if (inverse) sn = -sn;
+    const float nr = pr * c_s.x + pi * sn, ni = pi * c_s.x - pr * sn;  // (pr + i pi)(c - i sn)
+    pr = nr;
+    pi = ni;
+  }
+  return make_float2(pr, pi);
+}
+
+// Im Tr(Q M) = Im sum_ab Q_ab M_ba
+__device__ __forceinline__ float im_trace(const float *q, const float *m) {
+  // Q00 M00 + Q01 M10 + Q10 M01 + Q11 M11, imaginary parts
+  return (q[0] * m[1] + q[1] * m[0]) + (q[2] * m[5] + q[3] * m[4]) +
+         (q[4] * m[3] + q[5] * m[2]) + (q[6] * m[7] + q[7] * m[6]);
 }
 
 template <int KIND, int BIT, int NR>

@@ -1,5 +1,26 @@
 // Word format of a fused pass program (shared by the host planner and the kernels).
 //
+// v2 layout (qs_fused_pass / qs_adjoint_pass):
+//
+//   for each sub-pass:
+//     header (56 words): [0] number of segments, [6] number of phase entries,
+//       [1, 1+R) register local bits, [8, 8+R) swizzled LDS strides,
+//       [16, 16+R) global positions of the register bits, [24, 24+TB) thread-bit
+//       local bits, [40, 40+TB) their global positions
+//     phase entries, 4 words each: global position, cs index, flags
+//       (QS_SEG_PER_SAMPLE; QS_SEG_NOUNAPPLY = gradient probe only), gradient row
+//     segments:
+//       word: mask of register bits with a unitary | (number of CNOTs << 8)
+//       per masked bit, ascending: utab index, flags, nconst, qtab index, then
+//         nconst gradient rows (one per fused rotation, application order)
+//       per CNOT, 4 words: opcode (QS_OP_CNOT / QS_OP_CNOT_UNIFORM), target
+//         register bit, control register bit or global position, 0
+//   The forward runs each segment's unitaries then its CNOTs, and the phase
+//   entries last; the adjoint encoding lists sub-passes, segments and CNOTs in
+//   reverse, and the kernel un-applies phases first, then per segment the CNOTs
+//   and the unitaries.
+//
+// v1 layout (kept for reference in this comment only):
 // A pass is a stream of int32 words in device memory:
 //
 //   for each sub-pass:
@@ -33,6 +54,8 @@
 #define QS_OP_PAULI_GRAD 6
 #define QS_OP_PER_SAMPLE 0x100
 #define QS_PAULI_UNIFORM 0x10
+#define QS_SEG_PER_SAMPLE 1
+#define QS_SEG_NOUNAPPLY 2
 
 #define QS_MAX_LOCAL_BITS 16
 #define QS_MAX_QUBITS 40

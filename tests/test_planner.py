@@ -11,7 +11,7 @@ import pytest
 
 from oracle import fp64
 from qandle_amd._engine import planner as PL
-from tests.fused_emulator import emulate_program
+from tests.fused_emulator import emulate_adjoint, emulate_program
 
 KIND = {"rx": PL.RX, "ry": PL.RY, "rz": PL.RZ}
 
@@ -128,3 +128,35 @@ def test_pass_count_c2_family():
     stats = PL.plan_stats(pl, gates)
     print(f"C2 family: {len(gates)} gates in {len(pl.passes)} passes;", [s["gates"] for s in stats])
     assert len(pl.passes) <= 60
+
+
+@pytest.mark.parametrize("n,B,R,TB,mode", [(6, 3, 3, 4, "ladder"), (10, 2, 3, 4, "random"), (9, 2, 2, 3, "ladder")])
+def test_emulated_adjoint_matches_fp64(n, B, R, TB, mode):
+    """v2 adjoint words + composed gradient generators reproduce the FP64 adjoint."""
+    rng = np.random.default_rng(n * 7 + B)
+    ops = random_ops(rng, n, 2, B, mode)
+    gates, cs, efac = lower(n, ops, B)
+    cm = PL.CostModel(budget=40.0, min_low=2)
+    state, probs, pl = emulate_program(gates, n, B, R, TB, cs, efac, cm=cm)
+    w = rng.normal(size=(B, n))
+    grads, psi0, lam0 = emulate_adjoint(gates, n, B, R, TB, cs, state, w, pl,
+                                        embed_bits=[n - 1 - q for q in range(n)], embed_rot=0)
+    ref = fp64.adjoint_grads(n, ops, w)
+    rot_ops = [op for op in ops[1:] if op[0] != "cnot"]
+    rot_ref = [g for g, op in zip(ref[1:], ops[1:]) if op[0] != "cnot"]
+    for r, (op, gr) in enumerate(zip(rot_ops, rot_ref)):
+        got = grads[r].sum()
+        np.testing.assert_allclose(got, gr, atol=1e-10)
+    nrot = len(rot_ops)
+    gx = np.stack([grads[nrot + j] for j in range(n)], 1)
+    np.testing.assert_allclose(gx, ref[0], atol=1e-10)
+    np.testing.assert_allclose(psi0.reshape(B, -1), fp64.simulate(n, ops[:1]), atol=1e-10)
+
+
+def test_segmenter_fuses_runs_and_splits_at_cnots():
+    gates = [PL.Gate(PL.RX, 0, -1, 0), PL.Gate(PL.RY, 0, -1, 1), PL.Gate(PL.CNOT, 1, 0),
+             PL.Gate(PL.RZ, 1, -1, 2), PL.Gate(PL.RX, 0, -1, 3), PL.Gate(PL.RZ, 7, -1, 4)]
+    phases, segs = PL.segment_subpass(PL.SubPass([0, 1], list(range(6))), gates)
+    assert phases == [5]
+    assert segs[0].bits == {0: [0, 1]} and segs[0].cnots == [2]
+    assert segs[1].bits == {1: [3], 0: [4]} and segs[1].cnots == []
