@@ -38,7 +38,41 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--quick", action="store_true", help="headline only (no extra workloads, no CPU baseline)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", choices=["fused", "adjoint", "copy"],
+                    help="run only the named workload a few times (for rocprofv3 counter and trace runs)")
     return ap.parse_args()
+
+
+def profile_workload(mode, reps=2):
+    """Short fixed runs for profilers: the headline forward, its forward + adjoint
+    backward, or a device copy of the same state size (the HBM counter calibration)."""
+    import torch
+
+    import qandle_amd as qandle
+    from qandle_amd import workloads
+
+    dev = torch.device("cuda:0")
+    n, B = 20, 64
+    if mode == "copy":
+        a = torch.empty(B * 2 ** n, dtype=torch.complex64, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(reps + 1):
+            b.copy_(a)
+        torch.cuda.synchronize(dev)
+        print(f"copy: {a.numel() * 8} bytes read and written per launch", flush=True)
+        return
+    w = workloads.rxryrz_ladder(n=n, B=B, depth=10, seed=1)
+    circ = qandle.Circuit(w.layers, num_qubits=n).to(dev)
+    x = w.inputs["x"].to(dev)
+    for _ in range(reps):
+        if mode == "fused":
+            with torch.no_grad():
+                circ(x=x)
+        else:
+            xg = x.clone().requires_grad_(True)
+            circ(x=xg).sum().backward()
+    torch.cuda.synchronize(dev)
+    print(f"{mode}: {reps} runs of {w.name}", flush=True)
 
 
 def fused_bytes_per_launch(B, n, npass, stores_last):
@@ -58,6 +92,9 @@ def fused_bytes_per_launch(B, n, npass, stores_last):
 
 def main():
     args = parse()
+    if args.profile:
+        profile_workload(args.profile)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build targets a single MI355X "
               "(multi-GPU sharding is designed, not built; see DESIGN.md).", file=sys.stderr)

@@ -221,6 +221,37 @@ int qs_adjoint_finish_desc(void *desc, float *out, float *tot,
 int qs_items_adjoint_finish(const void *descs, int ndesc, int row_blocks,
                             void *stream);
 
+/* ---- run-time specialised pass kernels -----------------------------------
+ * Same role as qs_fused_pass (the executor loop of UnsplittedCircuit.forward,
+ * src/qandle/qcircuit.py:163-174, and the per-gate `state @ M` it runs,
+ * src/qandle/operators.py:289-298, 557-558), with the pass program compiled
+ * into the kernel: the host emits the HIP source of one kernel per pass, these
+ * calls compile it with hipRTC for the current device, load the code object
+ * and launch its kernels.
+ *
+ * qs_jit_compile compiles `src` with the given compiler options; *handle
+ * receives the result (code object and log; release with
+ * qs_jit_release_code) even when compilation fails (QS_EHIP: read the log).
+ * qs_jit_load loads a code object (from qs_jit_code_copy or a cache) into the
+ * current device and qs_jit_function looks up an extern "C" kernel.
+ * qs_jit_resident_blocks gives the workgroups of `block` threads that can be
+ * resident at once on the current device (the grid of a grid-stride kernel).
+ * qs_jit_launch launches `function` with the by-value argument block `args`
+ * (the kernel's parameter layout, `args_size` bytes). */
+int qs_jit_compile(const char *src, const char *name, const char *const *options,
+                   int noptions, void **handle);
+int64_t qs_jit_code_size(const void *handle);
+int qs_jit_code_copy(const void *handle, void *out, int64_t size);
+int64_t qs_jit_log_size(const void *handle);
+int qs_jit_log_copy(const void *handle, char *out, int64_t size);
+void qs_jit_release_code(void *handle);
+int qs_jit_load(const void *code, int64_t size, void **module);
+int qs_jit_function(void *module, const char *name, void **function);
+int64_t qs_jit_resident_blocks(void *function, int block);
+int qs_jit_launch(void *function, unsigned grid, unsigned block, const void *args,
+                  int64_t args_size, void *stream);
+int qs_jit_unload(void *module);
+
 #ifdef __cplusplus
 }
 #endif

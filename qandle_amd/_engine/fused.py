@@ -333,7 +333,7 @@ def compose_tables(ws: WordSet, cs: torch.Tensor, generators: bool = True):
 
 
 def _encode(p, gates, pl, index, per_sample, tables, rows=None):
-    w = PL.encode_pass_v2(p, gates, pl.R, pl.TB, index, per_sample, tables,
+    w = PL.encode_pass(p, gates, pl.R, pl.TB, index, per_sample, tables,
                           reverse=rows is not None, row_of=rows)
     return w, len(p.subs), np.asarray(p.L, dtype=np.int32)
 
@@ -429,6 +429,34 @@ def product_factors(prog: Program, inp: Inputs) -> torch.Tensor:
     return efac
 
 
+def tiles_of(B: int, n: int, K: int) -> int:
+    """Tiles of one pass (as qs_fused_pass counts them)."""
+    if n >= K:
+        return B << (n - K)
+    spt = 1 << (K - n)
+    return (B + spt - 1) // spt
+
+
+def use_jit(n: int, R: int, TB: int) -> bool:
+    from .. import config
+
+    return bool(getattr(config, "SPECIALISE", False))
+
+
+def specialised_kernels(prog: Program, pl: PL.Plan, ws: WordSet, B: int, gen_first: bool):
+    """Run-time compiled kernels of this plan's passes (cached on the program)."""
+    from .. import config
+    from . import jit
+
+    opts = getattr(config, "SPECIALISE_OPTIONS", {}) or {}
+    key = (id(pl), B, gen_first, prog.measure, tuple(sorted(opts.items())))
+    hit = prog.words_cache.get(("jit",) + key)
+    if hit is None:
+        hit = jit.ProgramKernels(ws.passes, prog.n, pl.R, pl.TB, gen_first, prog.measure, **opts)
+        prog.words_cache[("jit",) + key] = hit
+    return hit
+
+
 def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
             cm: PL.CostModel = None, want_state: bool = True, checkpoints=None, events=None):
     """Runs every pass.  Returns (state or None, probs or None).
@@ -456,21 +484,28 @@ def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
         wb = lib.qs_fused_measure_work_bytes(B, n, R, TB)
         mpart = torch.empty(max(wb // 4, 1), dtype=torch.float32, device=inp.device)
     npass = len(words)
+    kern = specialised_kernels(prog, pl, ws, B, src is None) if use_jit(n, R, TB) else None
+    ntiles = tiles_of(B, n, R + TB)
     for i, (w, nsub, lpos) in enumerate(words):
         last = i == npass - 1
         store = 1 if (not last or want_state or not prog.measure) else 0
         if events is not None:
             ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
             ev[0].record(torch.cuda.current_stream(inp.device))
-        L.check(
-            lib.qs_fused_pass(
-                L.ptr(psi), None if src is None else L.ptr(src), B, n, R, TB,
-                lpos.ctypes.data_as(L.ctypes.POINTER(L.ctypes.c_int32)), L.ptr(w), nsub,
-                L.ptr(cs), L.ptr(utab), None if efac is None else L.ptr(efac),
-                L.ptr(mpart) if (last and mpart is not None) else None, store, st,
-            ),
-            "qs_fused_pass",
-        )
+        if kern is not None:
+            kern.launch(i, L.ptr(psi), None if src is None else L.ptr(src), L.ptr(utab), L.ptr(cs),
+                        None if efac is None else L.ptr(efac),
+                        L.ptr(mpart) if (last and mpart is not None) else None, B, ntiles, store, st)
+        else:
+            L.check(
+                lib.qs_fused_pass(
+                    L.ptr(psi), None if src is None else L.ptr(src), B, n, R, TB,
+                    lpos.ctypes.data_as(L.ctypes.POINTER(L.ctypes.c_int32)), L.ptr(w), nsub,
+                    L.ptr(cs), L.ptr(utab), None if efac is None else L.ptr(efac),
+                    L.ptr(mpart) if (last and mpart is not None) else None, store, st,
+                ),
+                "qs_fused_pass",
+            )
         if events is not None:
             ev[1].record(torch.cuda.current_stream(inp.device))
             events.append(ev)

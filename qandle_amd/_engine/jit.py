@@ -1,0 +1,424 @@
+"""Per-pass specialised forward kernels, compiled at run time.
+
+The interpreted pass kernel (``qs_fused_pass``) reads its program words while
+it runs: every segment head, table index and CNOT costs a dependent scalar
+load and a branch, and the few waves a 2^11-amplitude tile allows cannot hide
+that latency.  Here the same encoded words of a pass are turned into the HIP
+source of one kernel with everything constant:
+
+* the tile layout (local and non-local bit positions, the LDS swizzle strides
+  of every register), the sub-pass register/thread maps;
+* fused unitaries read at constant table offsets (scalar loads, issued at
+  once) and applied on the matrix cores (``v_mfma_f32_4x4x1f32``);
+* CNOTs between register bits become a renaming of registers at code
+  generation time (no instruction at all); CNOTs with a control outside the
+  registers become per-thread selects;
+* the RZ phases, the product-state generator and the MeasureProbability
+  partials with their positions fixed.
+
+The kernel keeps the interpreted kernel's tile pipeline (the next tile's HBM
+loads in flight during the current tile's arithmetic), its LDS layout, its
+table formats (``qs_compose_tables``) and its measurement partial layout
+(``qs_fused_measure_finish``), so both paths produce the same results on the
+same inputs.  Sources are compiled once per process with hipRTC through the
+C-ABI (``qs_jit_*``) and the code objects are cached on disk by source hash.
+"""
+from __future__ import annotations
+
+import ctypes
+import hashlib
+import os
+import struct
+import threading
+
+from . import lib as L
+from . import planner as PL
+
+PRELUDE = r"""
+#include <hip/hip_runtime.h>
+typedef long long i64;
+typedef float qs_f4 __attribute__((ext_vector_type(4)));
+struct JArgs {
+  float *state; const float *src; const int *utab; const int *cs; const float *efac; float *mpart;
+  i64 B; i64 ntiles; int store; int pad;
+};
+__device__ __forceinline__ int swz(int l) { int f = (l >> 5) ^ (l >> 10) ^ (l >> 15); return l ^ (f & 31); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// the composed tables are read-only while a pass runs: reading them through the
+// constant address space lets uniform reads use the scalar (SMEM) path
+typedef const __attribute__((address_space(4))) int cint;
+__device__ __forceinline__ cint *as_const(const int *p) { return (cint *)(p); }
+// lane coefficients of a fused unitary (8 floats as int bits at u) for
+// v_mfma_f32_4x4x1f32: A_k = U[lane & 3][k] of the real 4x4 form
+__device__ __forceinline__ void coeffs(const int *up, float (&m)[4]) {
+  cint *u = as_const(up);
+  const float a = __int_as_float(u[0]), b = __int_as_float(u[1]), c = __int_as_float(u[2]),
+              d = __int_as_float(u[3]), e = __int_as_float(u[4]), f = __int_as_float(u[5]),
+              g = __int_as_float(u[6]), h = __int_as_float(u[7]);
+  const int row = threadIdx.x & 3;
+  const bool r1 = row & 1, r2 = row & 2;
+  m[0] = r2 ? (r1 ? f : e) : (r1 ? b : a);
+  m[1] = r2 ? (r1 ? e : -f) : (r1 ? a : -b);
+  m[2] = r2 ? (r1 ? h : g) : (r1 ? d : c);
+  m[3] = r2 ? (r1 ? g : -h) : (r1 ? c : -d);
+}
+__device__ __forceinline__ void mfma_pair(float2 &x, float2 &y, const float (&m)[4]) {
+  qs_f4 acc = {0.f, 0.f, 0.f, 0.f};
+  acc = __builtin_amdgcn_mfma_f32_4x4x1f32(m[0], x.x, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_4x4x1f32(m[1], x.y, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_4x4x1f32(m[2], y.x, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_4x4x1f32(m[3], y.y, acc, 0, 0, 0);
+  x = make_float2(acc[0], acc[1]);
+  y = make_float2(acc[2], acc[3]);
+}
+// vector-ALU form (per-sample unitaries that may differ inside a 4-lane block)
+__device__ __forceinline__ void valu_pair(float2 &x, float2 &y, const int *u) {
+  const float a = __int_as_float(u[0]), b = __int_as_float(u[1]), c = __int_as_float(u[2]),
+              d = __int_as_float(u[3]), e = __int_as_float(u[4]), f = __int_as_float(u[5]),
+              g = __int_as_float(u[6]), h = __int_as_float(u[7]);
+  const float2 p = x, q = y;
+  x = make_float2(a * p.x - b * p.y + c * q.x - d * q.y, a * p.y + b * p.x + c * q.y + d * q.x);
+  y = make_float2(e * p.x - f * p.y + g * q.x - h * q.y, e * p.y + f * p.x + g * q.y + h * q.x);
+}
+"""
+
+ARGS_FMT = "<QQQQQQqqii"   # struct JArgs
+OPTIONS = ["--offload-arch=gfx950", "-O3", "-std=c++17",
+           "-mllvm", "-simplifycfg-sink-common=false",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _swz(l: int) -> int:
+    return PL.swz(l)
+
+
+def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, measure: bool,
+                prefetch: bool = True, waves: int = None) -> str:
+    """HIP source of one specialised forward pass kernel (extern "C" ``name``).
+
+    ``prefetch`` keeps the next tile's HBM loads in flight in registers while
+    the current tile computes; ``waves`` caps the registers at that many
+    waves per SIMD (None: the compiler's choice)."""
+    K = R + TB
+    T, NR = 1 << TB, 1 << R
+    multi = n < K
+    subs = PL.decode_pass(words, R, TB)
+    lpos = [int(x) for x in lpos]
+    npos = [] if multi else [p for p in range(n) if p not in lpos]
+    nn = len(npos)
+    o = []
+    e = o.append
+    wattr = f" __attribute__((amdgpu_waves_per_eu({waves})))" if waves else ""
+    e(f'extern "C" __global__ void __launch_bounds__({T}){wattr}')
+    e(f"{name}(JArgs A) {{")
+    e(f"  __shared__ __align__(16) float2 tile[{1 << K}];")
+    if measure and TB > 6:
+        e(f"  __shared__ float red[{(T // 64) * (K + 1)}];")
+    e("  const int tid = threadIdx.x;")
+    e(f"  const i64 total = A.B << {n};")
+    e("  i64 g_tid = 0;")
+    for i in range(TB):
+        e(f"  g_tid |= (i64)((tid >> {i}) & 1) << {lpos[i + 1]};")
+    goff = [sum(1 << lpos[TB + 1 + i] for i in range(R - 1) if (it >> i) & 1) for it in range(NR // 2)]
+    e(f"  float4 pre[{max(NR // 2, 1)}];")
+    e("  auto tile_base = [&](i64 t) -> i64 {")
+    if multi:
+        e(f"    return t << {K};")
+    else:
+        e(f"    i64 b = (t >> {nn}) << {n};")
+        for i, p in enumerate(npos):
+            e(f"    b |= ((t >> {i}) & 1) << {p};")
+        e("    return b;")
+    e("  };")
+    pf = prefetch and not gen
+    if not gen:
+        e("  auto fetch = [&](i64 base) {")
+        for it in range(NR // 2):
+            e(f"    {{ const i64 gi = base | g_tid | {goff[it]}LL;")
+            if multi:  # the last tile of a multi-sample pass can run past the batch
+                e(f"      pre[{it}] = gi < total ? reinterpret_cast<const float4 *>(A.src)[gi >> 1]"
+                  " : make_float4(0.f, 0.f, 0.f, 0.f); }")
+            else:
+                e(f"      pre[{it}] = reinterpret_cast<const float4 *>(A.src)[gi >> 1]; }}")
+        e("  };")
+    e("  i64 t = blockIdx.x;")
+    if pf:
+        e("  if (t < A.ntiles) fetch(tile_base(t));")
+    e("  for (; t < A.ntiles; t += gridDim.x) {")
+    e("    const i64 base = tile_base(t);")
+    if not gen and not pf:
+        e("    fetch(base);")
+    if not gen:
+        for it in range(NR // 2):
+            e(f"    {{ const int sl = swz(2 * ({it * T} + tid)); const float4 val = pre[{it}]; const bool sw = sl & 1;")
+            e("      *reinterpret_cast<float4 *>(&tile[sl & ~1]) = make_float4(sw ? val.z : val.x, sw ? val.w : val.y,"
+              " sw ? val.x : val.z, sw ? val.y : val.w); }")
+        e("    __syncthreads();")
+        if pf:
+            e("    { const i64 tn = t + gridDim.x; if (tn < A.ntiles) fetch(tile_base(tn)); }")
+    e(f"    const i64 block_sample = base >> {n};")
+    e(f"    float2 v[{NR}];")
+    for s, sub in enumerate(subs):
+        _sub_pass(e, sub, s, len(subs), n, R, TB, multi, gen and s == 0, measure and s == len(subs) - 1)
+    e("    if (A.store) {")
+    for it in range(NR // 2):
+        e(f"      {{ const i64 gi = base | g_tid | {goff[it]}LL; const int sl = swz(2 * ({it * T} + tid));")
+        e("        const float4 raw = *reinterpret_cast<const float4 *>(&tile[sl & ~1]); const bool sw = sl & 1;")
+        e("        const float4 val = make_float4(sw ? raw.z : raw.x, sw ? raw.w : raw.y, sw ? raw.x : raw.z,"
+          " sw ? raw.y : raw.w);")
+        if multi:
+            e("        if (gi < total) reinterpret_cast<float4 *>(A.state)[gi >> 1] = val; }")
+        else:
+            e("        reinterpret_cast<float4 *>(A.state)[gi >> 1] = val; }")
+    e("    }")
+    e("    __syncthreads();")
+    e("  }")
+    e("}")
+    return "\n".join(o)
+
+
+def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure):
+    K = R + TB
+    T, NR = 1 << TB, 1 << R
+    e(f"    {{  // sub-pass {s}")
+    e("      int gl = 0; i64 gbase = base;")
+    for i in range(TB):
+        e(f"      gl |= ((tid >> {i}) & 1) << {sub.G[i]}; gbase |= (i64)((tid >> {i}) & 1) << {sub.GG[i]};")
+    e(f"      const i64 sample = {'gbase >> ' + str(n) if multi else 'block_sample'};")
+    e("      const bool live = sample < A.B;")
+    e("      const i64 csample = live ? sample : 0;")
+    e("      const int lb = swz(gl);")
+    sw = [_swz(1 << sub.P[r]) for r in range(R)]
+    swj = []
+    for j in range(NR):
+        a = 0
+        for r in range(R):
+            if (j >> r) & 1:
+                a ^= sw[r]
+        swj.append(a)
+    if gen:
+        rmask = 0
+        for r in range(R):
+            rmask |= 1 << sub.GP[r]
+        e("      if (live) {")
+        e(f"        const float2 *E = reinterpret_cast<const float2 *>(A.efac) + sample * {2 * n};")
+        e("        float2 pf = make_float2(1.f, 0.f);")
+        for pos in range(n):
+            if not (rmask >> pos) & 1:
+                e(f"        pf = cmul(pf, E[{2 * pos} + ((gbase >> {pos}) & 1)]);")
+        e("        v[0] = pf;")
+        for r in range(R):
+            e(f"        {{ const float2 f0 = E[{2 * sub.GP[r]}], f1 = E[{2 * sub.GP[r] + 1}];")
+            for j in range(1 << r):
+                e(f"          v[{j | (1 << r)}] = cmul(v[{j}], f1); v[{j}] = cmul(v[{j}], f0);")
+            e("        }")
+        e("      } else {")
+        for j in range(NR):
+            e(f"        v[{j}] = make_float2(0.f, 0.f);")
+        e("      }")
+    else:
+        for j in range(NR):
+            e(f"      v[{j}] = tile[lb ^ {swj[j]}];")
+    phys = list(range(NR))
+    for bits, cnots in sub.segments:
+        for r, ui, ps in bits:
+            off = f"8 * ({ui} + csample)" if ps else f"{8 * ui}"
+            e("      {")
+            if ps and multi:
+                e(f"        const int *u = A.utab + {off};")
+                for j in range(NR):
+                    if not (j >> r) & 1:
+                        e(f"        valu_pair(v[{phys[j]}], v[{phys[j | (1 << r)]}], u);")
+            else:
+                e(f"        float m[4]; coeffs(A.utab + {off}, m);")
+                for j in range(NR):
+                    if not (j >> r) & 1:
+                        e(f"        mfma_pair(v[{phys[j]}], v[{phys[j | (1 << r)]}], m);")
+            e("      }")
+        for uniform, tgt, ctl in cnots:
+            if not uniform:
+                for j in range(NR):
+                    if (j >> ctl) & 1 and not (j >> tgt) & 1:
+                        k = j | (1 << tgt)
+                        phys[j], phys[k] = phys[k], phys[j]
+            else:
+                e(f"      {{ const bool cnd = (gbase >> {ctl}) & 1;")
+                for j in range(NR):
+                    if not (j >> tgt) & 1:
+                        a, b = phys[j], phys[j | (1 << tgt)]
+                        e(f"        {{ const float2 x = v[{a}], y = v[{b}]; v[{a}] = cnd ? y : x; v[{b}] = cnd ? x : y; }}")
+                e("      }")
+    if sub.phases:
+        e("      {")
+        e("        float pr = 1.f, pi = 0.f;")
+        for pos, ci, fl in sub.phases:
+            if fl & PL.SEG_NOUNAPPLY:
+                continue
+            idx = f"({ci} + csample)" if fl & PL.SEG_PER_SAMPLE else f"{ci}"
+            e(f"        {{ const float c = __int_as_float(as_const(A.cs)[2 * {idx}]), "
+              f"sn0 = __int_as_float(as_const(A.cs)[2 * {idx} + 1]);")
+            e(f"          const float sn = ((gbase >> {pos}) & 1) ? -sn0 : sn0;")
+            e("          const float nr = pr * c + pi * sn, ni = pi * c - pr * sn; pr = nr; pi = ni; }")
+        e("        const float2 z = make_float2(pr, pi);")
+        for j in range(NR):
+            e(f"        v[{j}] = cmul(z, v[{j}]);")
+        e("      }")
+    if measure:
+        _measure(e, sub, phys, n, R, TB, multi)
+    for j in range(NR):
+        e(f"      tile[lb ^ {swj[j]}] = v[{phys[j]}];")
+    e("      __syncthreads();")
+    e("    }")
+
+
+def _measure(e, sub, phys, n, R, TB, multi):
+    """MeasureProbability partials of the tile (layout of qs_fused_measure_finish)."""
+    K = R + TB
+    T, NR = 1 << TB, 1 << R
+    e("      {")
+    e(f"        float p[{NR}]; float tot = 0.f;")
+    for j in range(NR):
+        e(f"        p[{j}] = v[{phys[j]}].x * v[{phys[j]}].x + v[{phys[j]}].y * v[{phys[j]}].y; tot += p[{j}];")
+    e(f"        float c[{K + 1}];")
+    for r in range(R):
+        terms = " + ".join(f"p[{j}]" for j in range(NR) if not (j >> r) & 1)
+        e(f"        c[{r}] = {terms};")
+    for i in range(TB):
+        e(f"        c[{R + i}] = ((tid >> {i}) & 1) ? 0.f : tot;")
+    e(f"        c[{K}] = tot;")
+    seglog = (n - R) if multi else TB
+    wlog = min(seglog, 6)
+    e(f"        for (int q = 0; q <= {K}; ++q) {{ float x = c[q];")
+    e(f"          for (int off = 1; off < {1 << wlog}; off <<= 1) x += __shfl_xor(x, off, 64); c[q] = x; }}")
+    spt_log = (K - n) if multi else 0
+    e(f"        float *mp = A.mpart + (t << {spt_log}) * {K + 1};")
+    if seglog <= 6:
+        e(f"        if ((tid & {(1 << seglog) - 1}) == 0) {{")
+        e(f"          float *ob = mp + (tid >> {seglog}) * {K + 1};")
+        for r in range(R):
+            e(f"          ob[{sub.P[r]}] = c[{r}];")
+        for i in range(TB):
+            e(f"          ob[{sub.G[i]}] = c[{R + i}];")
+        e(f"          ob[{K}] = c[{K}];")
+        e("        }")
+    else:
+        wps = 1 << (seglog - 6)
+        nsamp = (T // 64) // wps
+        dst = [sub.P[q] if q < R else (sub.G[q - R] if q < K else K) for q in range(K + 1)]
+        e("        if ((tid & 63) == 0) {")
+        e(f"          for (int q = 0; q <= {K}; ++q) red[(tid >> 6) * {K + 1} + q] = c[q];")
+        e("        }")
+        e("        __syncthreads();")
+        e(f"        if (tid < {nsamp * (K + 1)}) {{")
+        e(f"          const int sm = tid / {K + 1}, q = tid % {K + 1};")
+        e("          float x = 0.f;")
+        e(f"          for (int k = 0; k < {wps}; ++k) x += red[(sm * {wps} + k) * {K + 1} + q];")
+        e(f"          const int dst[{K + 1}] = {{{', '.join(str(d) for d in dst)}}};")
+        e(f"          mp[sm * {K + 1} + dst[q]] = x;")
+        e("        }")
+    e("      }")
+
+
+# ---------------------------------------------------------------------------
+# compile / cache / launch
+# ---------------------------------------------------------------------------
+
+_lock = threading.Lock()
+_modules: dict = {}          # source hash -> (module handle, {name: function})
+CACHE_DIR = os.environ.get("QANDLE_AMD_JIT_CACHE",
+                           os.path.join(os.path.expanduser("~"), ".cache", "qandle_amd", "jit"))
+
+
+def _compile(src: str, name: str) -> bytes:
+    lib = L.load()
+    opts = (ctypes.c_char_p * len(OPTIONS))(*[o.encode() for o in OPTIONS])
+    h = ctypes.c_void_p()
+    rc = lib.qs_jit_compile(src.encode(), (name + ".hip").encode(), opts, len(OPTIONS), ctypes.byref(h))
+    try:
+        if rc != 0:
+            ls = lib.qs_jit_log_size(h) if h.value else 0
+            log = ""
+            if ls and ls > 0:
+                buf = ctypes.create_string_buffer(int(ls) + 1)
+                lib.qs_jit_log_copy(h, buf, int(ls) + 1)
+                log = buf.value.decode(errors="replace")
+            raise L.EngineError(f"hipRTC compilation of {name} failed:\n{log[-4000:]}")
+        size = lib.qs_jit_code_size(h)
+        buf = ctypes.create_string_buffer(int(size))
+        L.check(lib.qs_jit_code_copy(h, buf, int(size)), "qs_jit_code_copy")
+        return buf.raw
+    finally:
+        if h.value:
+            lib.qs_jit_release_code(h)
+
+
+def module_for(src: str, names):
+    """Loaded module holding the kernels ``names`` of ``src`` (compiled or from the disk cache)."""
+    key = hashlib.sha256((PRELUDE + src + " ".join(OPTIONS)).encode()).hexdigest()[:32]
+    with _lock:
+        hit = _modules.get(key)
+        if hit is not None:
+            return hit
+        path = os.path.join(CACHE_DIR, key + ".hsaco")
+        code = None
+        try:
+            with open(path, "rb") as f:
+                code = f.read()
+        except OSError:
+            pass
+        if not code:
+            code = _compile(PRELUDE + src, "qs_jit_" + key)
+            try:
+                os.makedirs(CACHE_DIR, exist_ok=True)
+                tmp = path + f".{os.getpid()}.tmp"
+                with open(tmp, "wb") as f:
+                    f.write(code)
+                os.replace(tmp, path)
+            except OSError:
+                pass
+        lib = L.load()
+        mod = ctypes.c_void_p()
+        L.check(lib.qs_jit_load(code, len(code), ctypes.byref(mod)), "qs_jit_load")
+        funcs = {}
+        for nm in names:
+            fn = ctypes.c_void_p()
+            L.check(lib.qs_jit_function(mod, nm.encode(), ctypes.byref(fn)), f"qs_jit_function({nm})")
+            funcs[nm] = fn
+        hit = (mod, funcs)
+        _modules[key] = hit
+        return hit
+
+
+class ProgramKernels:
+    """The specialised kernels of every pass of one encoded forward program."""
+
+    def __init__(self, passes, n: int, R: int, TB: int, gen_first: bool, measure: bool,
+                 prefetch: bool = True, waves: int = None):
+        self.n, self.R, self.TB = n, R, TB
+        self.block = 1 << TB
+        self.names = []
+        parts = []
+        npass = len(passes)
+        for i, (w, _nsub, lpos) in enumerate(passes):
+            words = w.cpu().numpy() if hasattr(w, "cpu") else w
+            nm = f"qs_pass_{i}"
+            parts.append(pass_source(nm, words, n, R, TB, lpos, gen_first and i == 0, measure and i == npass - 1,
+                                     prefetch, waves))
+            self.names.append(nm)
+        self.source = "\n\n".join(parts)
+        self.mod, funcs = module_for(self.source, self.names)
+        self.funcs = [funcs[nm] for nm in self.names]
+        self.grid_cap = [None] * len(self.funcs)
+
+    def launch(self, i: int, state, src, utab, cs, efac, mpart, B: int, ntiles: int, store: int, stream):
+        lib = L.load()
+        if self.grid_cap[i] is None:
+            g = lib.qs_jit_resident_blocks(self.funcs[i], self.block)
+            self.grid_cap[i] = int(g) if g and g > 0 else 2048
+        grid = max(1, min(ntiles, self.grid_cap[i]))
+        args = struct.pack(ARGS_FMT, state or 0, src or 0, utab or 0, cs or 0, efac or 0, mpart or 0,
+                           B, ntiles, store, 0)
+        buf = ctypes.create_string_buffer(args, len(args))
+        L.check(lib.qs_jit_launch(self.funcs[i], grid, self.block, buf, len(args), stream), "qs_jit_launch")

@@ -41,6 +41,17 @@ SIGNATURES = {
     "qs_fused_supported": (I32, [I32, I32]),
     "qs_product_factors": (I32, [P, P, I64, I32, P, I32, I32, I64, P]),
     "qs_fused_pass": (I32, [P, P, I64, I32, I32, I32, P, P, I32, P, P, P, P, I32, P]),
+    "qs_jit_compile": (I32, [ctypes.c_char_p, ctypes.c_char_p, P, I32, P]),
+    "qs_jit_code_size": (I64, [P]),
+    "qs_jit_code_copy": (I32, [P, P, I64]),
+    "qs_jit_log_size": (I64, [P]),
+    "qs_jit_log_copy": (I32, [P, P, I64]),
+    "qs_jit_release_code": (None, [P]),
+    "qs_jit_load": (I32, [P, I64, P]),
+    "qs_jit_function": (I32, [P, ctypes.c_char_p, P]),
+    "qs_jit_resident_blocks": (I64, [P, I32]),
+    "qs_jit_launch": (I32, [P, ctypes.c_uint, ctypes.c_uint, P, I64, P]),
+    "qs_jit_unload": (I32, [P]),
     "qs_compose_tables": (I32, [P, P, P, P, I32, P, P]),
     "qs_fused_measure_work_bytes": (I64, [I64, I32, I32, I32]),
     "qs_fused_measure_finish": (I32, [P, P, I64, I32, I32, I32, P, P]),

@@ -29,13 +29,11 @@ import typing
 
 import numpy as np
 
-# op kinds (match qs_program.h for rotations)
+# gate kinds (rotations match the QS_RX/RY/RZ codes of the C-ABI)
 RX, RY, RZ, CNOT = 0, 1, 2, 4
 # gradient-only Pauli reductions (adjoint kernel): kind = PGRAD + Pauli (0 X, 1 Y, 2 Z)
 PGRAD = 8
-OP_RX, OP_RY, OP_RZ, OP_RZ_UNIFORM, OP_CNOT, OP_CNOT_UNIFORM, OP_PAULI_GRAD = 0, 1, 2, 3, 4, 5, 6
-OP_PER_SAMPLE = 0x100
-PAULI_UNIFORM = 0x10
+CNOT_UNIFORM = 1      # CNOT word flag: control outside the registers (qs_program.h)
 
 HDR = 56
 HDR_P, HDR_SWZ, HDR_GP, HDR_G, HDR_GG = 1, 8, 16, 24, 40
@@ -211,61 +209,6 @@ def swz(l: int) -> int:
     return l ^ (f & 31)
 
 
-def encode_pass(p: Pass, gates, R: int, TB: int, coef_index, per_sample,
-                reverse: bool = False, row_of=None) -> np.ndarray:
-    """int32 words of one pass (layout in qs_program.h / qs_fused.hip).
-
-    ``coef_index(slot)`` gives the float2 index of a rotation's coefficient
-    (for a per-sample slot: the index of sample 0); ``per_sample(slot)`` says
-    whether it advances with the sample index.  ``reverse`` encodes the pass
-    for the adjoint kernel: sub-passes and the gates inside each in reverse
-    order, and each rotation carries its gradient row ``row_of(gate)`` in
-    word b.
-    """
-    K = R + TB
-    L = p.L
-    loc = {pos: j for j, pos in enumerate(L)}
-    words = []
-    for sp in (reversed(p.subs) if reverse else p.subs):
-        Pl = [loc[q] for q in sp.P]                     # register local bits
-        G = [j for j in range(K) if j not in Pl]        # thread-bit local bits
-        hdr = [0] * HDR
-        hdr[0] = len(sp.gates)
-        for r, j in enumerate(Pl):
-            hdr[HDR_P + r] = j
-            hdr[HDR_SWZ + r] = swz(1 << j)
-            hdr[HDR_GP + r] = L[j]
-        for i, j in enumerate(G):
-            hdr[HDR_G + i] = j
-            hdr[HDR_GG + i] = L[j]
-        words.extend(hdr)
-        reg = {q: r for r, q in enumerate(sp.P)}
-        for gi in (reversed(sp.gates) if reverse else sp.gates):
-            g = gates[gi]
-            if g.kind >= PGRAD:
-                pauli = g.kind - PGRAD
-                if g.pos in reg:
-                    words.extend([OP_PAULI_GRAD, reg[g.pos], row_of(gi), pauli])
-                else:
-                    words.extend([OP_PAULI_GRAD, g.pos, row_of(gi), pauli | PAULI_UNIFORM])
-                continue
-            if g.kind == CNOT:
-                if g.ctrl in reg:
-                    words.extend([OP_CNOT, reg[g.pos], reg[g.ctrl], 0])
-                else:
-                    words.extend([OP_CNOT_UNIFORM, reg[g.pos], g.ctrl, 0])
-                continue
-            code = g.kind if g.pos in reg else OP_RZ_UNIFORM
-            if code == OP_RZ_UNIFORM and g.kind != RZ:
-                raise AssertionError("rotation scheduled off its register bit")
-            if per_sample(g.slot):
-                code |= OP_PER_SAMPLE
-            a = reg[g.pos] if g.pos in reg else g.pos
-            b = row_of(gi) if reverse else 0
-            words.extend([code, a, b, int(coef_index(g.slot))])
-    return np.asarray(words, dtype=np.int32)
-
-
 def plan_stats(pl: Plan, gates, cm: CostModel = None):
     cm = cm or CostModel()
     out = []
@@ -277,7 +220,7 @@ def plan_stats(pl: Plan, gates, cm: CostModel = None):
 
 
 # ----------------------------------------------------------------------------
-# v2 op format: per sub-pass, phase entries + segments of fused per-bit unitaries
+# word encoding (layout in qs_program.h)
 # ----------------------------------------------------------------------------
 #
 # Single-qubit gates on different bits commute, and every RZ on a bit that is
@@ -292,8 +235,6 @@ def plan_stats(pl: Plan, gates, cm: CostModel = None):
 
 SEG_NOUNAPPLY = 2
 SEG_PER_SAMPLE = 1
-
-
 @dataclasses.dataclass
 class Segment:
     bits: dict                 # register index -> [gate indices, in application order]
@@ -363,15 +304,18 @@ class TableBuilder:
         return np.asarray(self.words if self.words else [0], dtype=np.int32)
 
 
-def encode_pass_v2(p: Pass, gates, R: int, TB: int, coef_index, per_sample, tables: TableBuilder,
-                   reverse: bool = False, row_of=None, B: int = None) -> np.ndarray:
+def encode_pass(p: Pass, gates, R: int, TB: int, coef_index, per_sample, tables: TableBuilder,
+                reverse: bool = False, row_of=None, B: int = None) -> np.ndarray:
     """int32 words of one pass in the layout of qs_program.h.
 
-    Forward segments list their per-bit unitaries, then their CNOTs (the
-    order they apply in).  ``reverse`` encodes for the adjoint kernel:
-    sub-passes, segments and CNOTs in reverse order, each segment's CNOT
-    words before its unitary entries (the order they are undone in), and
-    every rotation's gradient row ``row_of(gate)`` after its entry."""
+    ``coef_index(slot)`` is the float2 index of a rotation's (cos, sin) entry
+    (for a per-sample slot: the entry of sample 0) and ``per_sample(slot)``
+    says whether it advances with the sample.  Fused unitaries are allocated
+    in ``tables``.  Forward segments are fixed records (head, one utab index
+    per register bit) followed by their CNOT words.  ``reverse`` encodes for
+    the adjoint kernel: sub-passes, segments and CNOTs in reverse order, each
+    segment's CNOT words before its unitary entries (the order they are undone
+    in), and every rotation's gradient row ``row_of(gate)``."""
     K = R + TB
     L = p.L
     loc = {pos: j for j, pos in enumerate(L)}
@@ -399,17 +343,13 @@ def encode_pass_v2(p: Pass, gates, R: int, TB: int, coef_index, per_sample, tabl
         for gi in phases:
             g = gates[gi]
             if g.kind >= PGRAD:  # gradient probe only (Z on a non-register bit)
-                words.extend([g.pos, 0, 2, rows(gi)])
+                words.extend([g.pos, 0, SEG_NOUNAPPLY, rows(gi)])
             else:
-                words.extend([g.pos, int(coef_index(g.slot)), 1 if per_sample(g.slot) else 0, rows(gi)])
+                words.extend([g.pos, int(coef_index(g.slot)), SEG_PER_SAMPLE if per_sample(g.slot) else 0, rows(gi)])
         for seg in segs:
             cn = list(reversed(seg.cnots)) if reverse else seg.cnots
-            umask = 0
-            for r in seg.bits:
-                umask |= 1 << r
-            words.append(umask | (len(cn) << 8))
-            if reverse:  # the adjoint undoes the segment's CNOTs before its unitaries
-                _cnot_words(words, cn, gates, reg)
+            umask = psmask = probemask = 0
+            entries = {}
             for r in sorted(seg.bits):
                 gl = seg.bits[r]
                 consts = []
@@ -422,18 +362,66 @@ def encode_pass_v2(p: Pass, gates, R: int, TB: int, coef_index, per_sample, tabl
                     else:
                         consts.append((g.kind, int(coef_index(g.slot)), bool(per_sample(g.slot))))
                 u, ps, q = tables.add(consts, B)
-                flags = (SEG_PER_SAMPLE if ps else 0) | (SEG_NOUNAPPLY if probe else 0)
-                words.extend([u, flags, len(gl), q])
-                words.extend(rows(gi) for gi in gl)
-            if not reverse:
-                _cnot_words(words, cn, gates, reg)
+                umask |= 1 << r
+                psmask |= (1 << r) if ps else 0
+                probemask |= (1 << r) if probe else 0
+                entries[r] = (u, q, gl)
+            words.append(umask | (len(cn) << 8) | (psmask << 16) | (probemask << 24))
+            cwords = [_cnot_word(gates[gi], reg) for gi in cn]
+            if reverse:
+                words.extend(cwords)
+                for r in sorted(entries):
+                    u, q, gl = entries[r]
+                    words.extend([u, len(gl), q])
+                    words.extend(rows(gi) for gi in gl)
+            else:
+                words.extend(entries[r][0] if r in entries else 0 for r in range(R))
+                words.extend(cwords)
     return np.asarray(words, dtype=np.int32)
 
 
-def _cnot_words(words, cn, gates, reg):
-    for gi in cn:
-        g = gates[gi]
-        if g.ctrl in reg:
-            words.extend([OP_CNOT, reg[g.pos], reg[g.ctrl], 0])
-        else:
-            words.extend([OP_CNOT_UNIFORM, reg[g.pos], g.ctrl, 0])
+def _cnot_word(g: Gate, reg) -> int:
+    if g.ctrl in reg:
+        return (reg[g.pos] << 8) | (reg[g.ctrl] << 16)
+    return CNOT_UNIFORM | (reg[g.pos] << 8) | (g.ctrl << 16)
+
+
+@dataclasses.dataclass
+class DecodedSub:
+    P: list          # register bit r -> tile-local bit
+    G: list          # thread bit i -> tile-local bit
+    GP: list         # register bit r -> global position
+    GG: list         # thread bit i -> global position
+    phases: list     # (global position, cs index, flags)
+    segments: list   # ([(r, utab index, per_sample)], [(uniform, target register, control)])
+
+
+def decode_pass(words, R: int, TB: int):
+    """Forward words of one pass -> sub-pass structures (the layout of qs_program.h)."""
+    w = [int(x) for x in words]
+    pos, out = 0, []
+    while pos < len(w):
+        hdr = w[pos: pos + HDR]
+        nseg, nph = hdr[0], hdr[6]
+        sub = DecodedSub([hdr[HDR_P + r] for r in range(R)], [hdr[HDR_G + i] for i in range(TB)],
+                         [hdr[HDR_GP + r] for r in range(R)], [hdr[HDR_GG + i] for i in range(TB)], [], [])
+        p = pos + HDR
+        for _ in range(nph):
+            sub.phases.append((w[p], w[p + 1], w[p + 2]))
+            p += 4
+        for _ in range(nseg):
+            head = w[p]
+            umask, ncnot, psmask = head & 0xFF, (head >> 8) & 0xFF, (head >> 16) & 0xFF
+            idx = w[p + 1: p + 1 + R]
+            p += 1 + R
+            bits = [(r, idx[r], bool((psmask >> r) & 1)) for r in range(R) if (umask >> r) & 1]
+            cn = []
+            for cw in w[p: p + ncnot]:
+                cn.append((bool(cw & CNOT_UNIFORM), (cw >> 8) & 0xFF, cw >> 16))
+            p += ncnot
+            sub.segments.append((bits, cn))
+        out.append(sub)
+        pos = p
+    if pos != len(w):
+        raise ValueError("malformed pass words")
+    return out

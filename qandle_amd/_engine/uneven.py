@@ -183,7 +183,7 @@ class UnevenBatch:
         tables = PL.TableBuilder(1)
         for c in self.circs:
             for p in c.plan.passes:
-                w = PL.encode_pass_v2(p, c.prog.gates, R, TB, lambda s, c=c: c.phi_off + c.prog.slot_index[s],
+                w = PL.encode_pass(p, c.prog.gates, R, TB, lambda s, c=c: c.phi_off + c.prog.slot_index[s],
                                       lambda s: False, tables)
                 chunks.append(w)
                 where.append(pos)
@@ -247,12 +247,12 @@ class UnevenBatch:
                     rows[gi] = len(rows)
             seq = []
             for p in reversed(c.plan.passes):
-                w = PL.encode_pass_v2(p, c.prog.gates, R, TB, lambda s, c=c: c.phi_off + c.prog.slot_index[s],
+                w = PL.encode_pass(p, c.prog.gates, R, TB, lambda s, c=c: c.phi_off + c.prog.slot_index[s],
                                       lambda s: False, tables, reverse=True, row_of=rows.__getitem__)
                 seq.append((p, w))
             if c.gplan is not None:
                 for p in c.gplan.passes:
-                    w = PL.encode_pass_v2(p, c.ggates, R, TB, lambda s: 0, lambda s: False, tables,
+                    w = PL.encode_pass(p, c.ggates, R, TB, lambda s: 0, lambda s: False, tables,
                                           reverse=True, row_of=lambda gi, c=c: c.nrot + gi)
                     seq.append((p, w))
             st = []

@@ -1,4 +1,7 @@
-"""Build libqandle_sv.so for gfx950 with hipcc (in-tree, no JIT cache).
+"""Build libqandle_sv.so for gfx950 with hipcc (in-tree).
+
+The library links hipRTC for the per-pass specialised kernels it compiles at
+run time (qs_jit_*); the kernels shipped in the library itself are built here.
 
     python -m qandle_amd.csrc.build
 """
@@ -45,7 +48,8 @@ def build(verbose: bool = False) -> str:
         subprocess.run(cmd, check=True)
         objs.append(obj)
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp,
+           "-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -23,9 +23,9 @@ namespace qs {
 struct AdjArgs {
   float *psi;
   float *lam;
-  const float *cs;
-  const float *utab;  // fused per-bit unitaries (un-applied as U^dagger)
-  const float *qtab;  // gradient generators of their constituents
+  const int32_t *cs;    // (cos, sin) float pairs, read as int32 bits
+  const int32_t *utab;  // fused per-bit unitaries (un-applied as U^dagger), int32 bits
+  const int32_t *qtab;  // gradient generators of their constituents, int32 bits
   float *gpart;
   const int32_t *words;
   const float *seed_w;  // non-null: lam = 2 psi sum_q w[b, q] [bit n-1-q = 0], (B, n) row-major
@@ -37,28 +37,6 @@ struct AdjArgs {
   int32_t lpos[QS_MAX_LOCAL_BITS];
   int32_t npos[QS_MAX_QUBITS];
 };
-
-template <int KIND, int BIT, int NR>
-__device__ __forceinline__ float im_reg(const float2 (&l)[NR], const float2 (&p)[NR]) {
-  float acc = 0.f;
-#pragma unroll
-  for (int j = 0; j < NR; ++j)
-    if (!(j & (1 << BIT))) acc += pauli_im_pair<KIND>(l[j], l[j | (1 << BIT)], p[j], p[j | (1 << BIT)]);
-  return acc;
-}
-
-template <int KIND, int R>
-__device__ __forceinline__ float im_dispatch(const float2 (&l)[1 << R], const float2 (&p)[1 << R], int bit) {
-  switch (bit) {
-    case 0: return im_reg<KIND, 0>(l, p);
-    case 1: if constexpr (R > 1) return im_reg<KIND, 1>(l, p); break;
-    case 2: if constexpr (R > 2) return im_reg<KIND, 2>(l, p); break;
-    case 3: if constexpr (R > 3) return im_reg<KIND, 3>(l, p); break;
-    case 4: if constexpr (R > 4) return im_reg<KIND, 4>(l, p); break;
-    default: break;
-  }
-  return 0.f;
-}
 
 template <int R, int TB, bool MULTI>
 __device__ __forceinline__ void adjoint_tile(const AdjArgs &A, const int64_t t, float2 *tp, float2 *tl) {
@@ -124,7 +102,6 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &A, const int64_t t, 
 
   float2 vp[NR], vl[NR];
   const int32_t *w = A.words;
-  const float2 *cs = reinterpret_cast<const float2 *>(A.cs);
   for (int s = 0; s < A.nsub; ++s) {
     const int nseg = w[0], nphase = w[6];
     int gl = 0;
@@ -168,7 +145,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &A, const int64_t t, 
 #pragma unroll
       for (int j = 0; j < NR; ++j) S += vl[j].x * vp[j].y - vl[j].y * vp[j].x;
       for (int k = 0; k < nphase; ++k) reduce_put(((gbase >> ph[4 * k]) & 1) ? -S : S, ph[4 * k + 3]);
-      const float2 z = phase_product(ph, nphase, cs, csample, gbase, true);
+      const float2 z = phase_product(ph, nphase, A.cs, csample, gbase, true);
 #pragma unroll
       for (int j = 0; j < NR; ++j) {
         vp[j] = cmul(z, vp[j]);
@@ -176,33 +153,37 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &A, const int64_t t, 
       }
     }
     for (int sg = 0; sg < nseg; ++sg) {
-      const int head = *op++;
-      const int umask = head & 0xff, ncnot = head >> 8;
-      for (int c = 0; c < ncnot; ++c, op += 4) {
-        if (op[0] == QS_OP_CNOT) {
-          cnot_dispatch<R>(vp, op[2], op[1]);
-          cnot_dispatch<R>(vl, op[2], op[1]);
-        } else if ((gbase >> op[2]) & 1) {
-          swap_dispatch<R>(vp, op[1]);
-          swap_dispatch<R>(vl, op[1]);
-        }
+      const int head = op[0];
+      const int umask = head & 0xff, ncnot = (head >> 8) & 0xff;
+      const int psmask = (head >> 16) & 0xff, probemask = (head >> 24) & 0x7f;
+      ++op;
+      for (int c = 0; c < ncnot; ++c) {
+        run_cnot<R>(vp, op[c], gbase);
+        run_cnot<R>(vl, op[c], gbase);
       }
+      op += ncnot;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if ((umask >> r) & 1) {
-          const int flags = op[1], nconst = op[2];
-          const int64_t var = (flags & QS_SEG_PER_SAMPLE) ? csample : 0;
-          const int64_t span = (flags & QS_SEG_PER_SAMPLE) ? A.B : 1;
-          float m[8];
-          cross_density_at<R>(vp, vl, r, m);
+          const int nconst = op[1];
+          const bool ps = (psmask >> r) & 1;
+          const int64_t var = ps ? csample : 0;
+          const int64_t span = ps ? A.B : 1;
+          const M8 m = cross_density_at<R>(vp, vl, r);
           for (int k = 0; k < nconst; ++k)
-            reduce_put(im_trace(A.qtab + 8 * (op[3] + k * span + var), m), op[4 + k]);
-          if (!(flags & QS_SEG_NOUNAPPLY)) {
-            const float *uu = A.utab + 8 * (op[0] + var);
-            apply_udag_at<R>(vp, r, uu);
-            apply_udag_at<R>(vl, r, uu);
+            reduce_put(im_trace(load_m8(A.qtab + 8 * (op[2] + k * span + var)), m), op[3 + k]);
+          if (!((probemask >> r) & 1)) {
+            const M8 uu = load_m8(A.utab + 8 * (op[0] + var));
+            if (MULTI && ps) {  // may differ inside a 4-lane matrix block
+              apply_valu_at<R>(vp, r, dagger(uu));
+              apply_valu_at<R>(vl, r, dagger(uu));
+            } else {
+              const MU ud = mfma_coeffs<true>(uu);
+              apply_mfma_at<R>(vp, r, ud);
+              apply_mfma_at<R>(vl, r, ud);
+            }
           }
-          op += 4 + nconst;
+          op += 3 + nconst;
         }
       }
     }
@@ -345,9 +326,9 @@ static int fill_adj_args(AdjArgs &A, int64_t *ntiles, float *psi, float *lam, in
   const int K = R + TB;
   A.psi = psi;
   A.lam = lam;
-  A.cs = cs;
-  A.utab = utab;
-  A.qtab = qtab;
+  A.cs = reinterpret_cast<const int32_t *>(cs);
+  A.utab = reinterpret_cast<const int32_t *>(utab);
+  A.qtab = reinterpret_cast<const int32_t *>(qtab);
   A.gpart = gpart;
   A.words = words;
   A.seed_w = seed_w;

@@ -20,6 +20,28 @@ inline int launch_status() {
   return QS_OK;
 }
 
+// Grid for a kernel that walks its work items in a grid-stride loop: every
+// workgroup that can be resident on the current device at once (from the
+// occupancy query, cached per kernel, block size and device), capped by the
+// number of items.  Host-side; allocates nothing.
+template <typename Kernel>
+unsigned resident_grid(Kernel kernel, int block, int64_t items) {
+  static int cached[64];  // per device: resident workgroups + 1 (0 = not queried)
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  int g = cached[dev] - 1;
+  if (g <= 0) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 1;
+    g = per_cu * cus;
+    cached[dev] = g + 1;
+  }
+  if (items < 1) return 1;
+  return (unsigned)(items < g ? items : g);
+}
+
 __device__ __forceinline__ float2 ld2(const float *p, int64_t i) {
   return reinterpret_cast<const float2 *>(p)[i];
 }

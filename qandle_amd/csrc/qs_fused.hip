@@ -27,8 +27,8 @@ namespace qs {
 struct PassArgs {
   float *state;
   const float *src;
-  const float *cs;
-  const float *utab;  // fused per-bit unitaries, 8 floats each
+  const int32_t *cs;    // (cos, sin) float pairs, read as int32 bits
+  const int32_t *utab;  // fused per-bit unitaries, 8 floats each, read as int32 bits
   const float *efac;
   float *mpart;
   const int32_t *words;
@@ -42,10 +42,65 @@ struct PassArgs {
   int32_t npos[QS_MAX_QUBITS];
 };
 
-// One tile of one pass.  `tile` (2^K float2, 16-byte aligned) and `red`
-// ((T/64)*(K+1) floats) are the caller's LDS.
+// Global index of tile t's first amplitude (its non-local bits).
+__device__ __forceinline__ int64_t tile_base(const PassArgs &A, int64_t t, int K, bool multi) {
+  if (multi) return t << K;
+  const int64_t tl = t & ((int64_t(1) << A.nn) - 1);
+  int64_t base = (t >> A.nn) << A.n;
+  for (int i = 0; i < A.nn; ++i)
+    if ((tl >> i) & 1) base |= int64_t(1) << A.npos[i];
+  return base;
+}
+
+// Offset of this thread's first load/store slot: thread bit i is local bit
+// i + 1 (local bit 0 is the amplitude pair inside one 16-byte vector).
+template <int TB>
+__device__ __forceinline__ int64_t thread_offset(const PassArgs &A, int tid) {
+  int64_t g = 0;
+#pragma unroll
+  for (int i = 0; i < TB; ++i)
+    if ((tid >> i) & 1) g |= int64_t(1) << A.lpos[i + 1];
+  return g;
+}
+
+// Issues the HBM loads of one tile into registers (slot it of this thread =
+// amplitude pair it * T + tid of the tile, in local-index order).  The loads
+// stay in flight until the registers are staged into LDS.
+template <int R, int TB>
+__device__ __forceinline__ void fetch_tile(const PassArgs &A, int64_t base, int64_t g_tid,
+                                           float4 (&pre)[1 << (R - 1)]) {
+  const int64_t total = A.B << A.n;
+#pragma unroll
+  for (int it = 0; it < (1 << (R - 1)); ++it) {
+    int64_t g_it = 0;
+#pragma unroll
+    for (int i = 0; i < R - 1; ++i)
+      if ((it >> i) & 1) g_it |= int64_t(1) << A.lpos[TB + 1 + i];
+    const int64_t gi = base | g_tid | g_it;
+    pre[it] = gi < total ? reinterpret_cast<const float4 *>(A.src)[gi >> 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+// Writes fetched registers to their swizzled LDS slots (an amplitude pair
+// shares one 16-byte slot, swapped when bit 0 of its swizzled index is set).
+template <int R, int TB>
+__device__ __forceinline__ void stage_tile(float2 *tile, const float4 (&pre)[1 << (R - 1)], int tid) {
+#pragma unroll
+  for (int it = 0; it < (1 << (R - 1)); ++it) {
+    const int sl = swz(2 * (it * (1 << TB) + tid));
+    float4 val = pre[it];
+    if (sl & 1) val = make_float4(val.z, val.w, val.x, val.y);
+    *reinterpret_cast<float4 *>(&tile[sl & ~1]) = val;
+  }
+}
+
+// Sub-passes of one tile on its LDS copy (staged by the caller when the pass
+// reads a state; generated in registers when it does not), then the store.
+// `tile` (2^K float2, 16-byte aligned) and `red` ((T/64)*(K+1) floats) are
+// the caller's LDS.
 template <int R, int TB, bool MULTI>
-__device__ __forceinline__ void pass_tile(const PassArgs &A, const int64_t t, float2 *tile, float *red) {
+__device__ __forceinline__ void pass_body(const PassArgs &A, const int64_t t, const int64_t base,
+                                          const int64_t g_tid, float2 *tile, float *red) {
   constexpr int T = 1 << TB;
   constexpr int K = R + TB;
   constexpr int NR = 1 << R;
@@ -53,39 +108,7 @@ __device__ __forceinline__ void pass_tile(const PassArgs &A, const int64_t t, fl
   const int tid = threadIdx.x;
   const int n = A.n;
   const int64_t total = A.B << n;
-  int64_t base;
-  if (MULTI) {
-    base = t << K;
-  } else {
-    const int64_t tl = t & ((int64_t(1) << A.nn) - 1);
-    base = (t >> A.nn) << n;
-    for (int i = 0; i < A.nn; ++i)
-      if ((tl >> i) & 1) base |= int64_t(1) << A.npos[i];
-  }
   const int64_t block_sample = base >> n;
-
-  // global offsets of this thread's load/store slots
-  int64_t g_tid = 0;
-#pragma unroll
-  for (int i = 0; i < TB; ++i)
-    if ((tid >> i) & 1) g_tid |= int64_t(1) << A.lpos[i + 1];
-
-  if (A.src) {
-#pragma unroll
-    for (int it = 0; it < (NR >> 1); ++it) {
-      int64_t g_it = 0;
-#pragma unroll
-      for (int i = 0; i < R - 1; ++i)
-        if ((it >> i) & 1) g_it |= int64_t(1) << A.lpos[TB + 1 + i];
-      const int64_t gi = base | g_tid | g_it;
-      float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (gi < total) val = reinterpret_cast<const float4 *>(A.src)[gi >> 1];
-      const int sl = swz(2 * (it * T + tid));
-      if (sl & 1) val = make_float4(val.z, val.w, val.x, val.y);
-      *reinterpret_cast<float4 *>(&tile[sl & ~1]) = val;
-    }
-    __syncthreads();
-  }
 
   float2 v[NR];
   const int32_t *w = A.words;
@@ -153,25 +176,32 @@ __device__ __forceinline__ void pass_tile(const PassArgs &A, const int64_t t, fl
     const int32_t *ph = w + kHdr;
     const int32_t *op = ph + 4 * nphase;
     for (int sg = 0; sg < nseg; ++sg) {
-      const int head = *op++;
-      const int umask = head & 0xff, ncnot = head >> 8;
+      const int head = op[0];
+      const int umask = head & 0xff, ncnot = (head >> 8) & 0xff, psmask = (head >> 16) & 0xff;
+      // every table read of the segment is issued before the first use
+      M8 u[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int64_t ui = op[1 + r] + (((psmask >> r) & 1) ? csample : 0);
+        u[r] = load_m8(A.utab + 8 * ui);
+      }
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if ((umask >> r) & 1) {
-          const int64_t ui = op[0] + ((op[1] & QS_SEG_PER_SAMPLE) ? csample : 0);
-          apply_u_at<R>(v, r, A.utab + 8 * ui);
-          op += 4 + op[2];
+          // per-sample unitaries of a multi-sample tile may differ inside a
+          // 4-lane matrix block: those stay on the vector ALU
+          if (MULTI && ((psmask >> r) & 1))
+            apply_valu_at<R>(v, r, u[r]);
+          else
+            apply_mfma_at<R>(v, r, mfma_coeffs<false>(u[r]));
         }
       }
-      for (int c = 0; c < ncnot; ++c, op += 4) {
-        if (op[0] == QS_OP_CNOT)
-          cnot_dispatch<R>(v, op[2], op[1]);
-        else if ((gbase >> op[2]) & 1)
-          swap_dispatch<R>(v, op[1]);
-      }
+      op += 1 + R;
+      for (int c = 0; c < ncnot; ++c) run_cnot<R>(v, op[c], gbase);
+      op += ncnot;
     }
     if (nphase) {
-      const float2 z = phase_product(ph, nphase, reinterpret_cast<const float2 *>(A.cs), csample, gbase, false);
+      const float2 z = phase_product(ph, nphase, A.cs, csample, gbase, false);
 #pragma unroll
       for (int j = 0; j < NR; ++j) v[j] = cmul(z, v[j]);
     }
@@ -265,27 +295,76 @@ __device__ __forceinline__ void pass_tile(const PassArgs &A, const int64_t t, fl
   }
 }
 
+// Waves per SIMD the pass kernels are register-limited to: two for 32
+// amplitudes per thread (the LDS tile allows about two and a half), three below;
+// one for the multi-sample form with 32 amplitudes per thread, which also
+// carries the vector-ALU path for per-sample unitaries.
+template <int R, bool MULTI = false>
+constexpr int kPassWaves = R >= 5 ? (MULTI ? 1 : 2) : 3;
+
+// One pass over `ntiles` tiles.  Each workgroup walks tiles t, t + G, ...
+// (G = gridDim.x, sized to the resident workgroups) and keeps the next tile's
+// HBM loads in flight while it computes the current one, so the memory
+// latency of a tile overlaps the arithmetic of the previous one.
 template <int R, int TB, bool MULTI>
-__global__ void __launch_bounds__(1 << TB) k_pass(PassArgs A) {
+__global__ void __launch_bounds__(1 << TB) __attribute__((amdgpu_waves_per_eu(kPassWaves<R, MULTI>)))
+k_pass(PassArgs A, int64_t ntiles) {
   __shared__ __align__(16) float2 tile[1 << (R + TB)];
   __shared__ float red[((1 << TB) / 64) * (R + TB + 1)];
-  pass_tile<R, TB, MULTI>(A, blockIdx.x, tile, red);
+  constexpr int K = R + TB;
+  const int tid = threadIdx.x;
+  const int64_t g_tid = thread_offset<TB>(A, tid);
+  float4 pre[1 << (R - 1)];
+  int64_t t = blockIdx.x;
+  if (A.src && t < ntiles) fetch_tile<R, TB>(A, tile_base(A, t, K, MULTI), g_tid, pre);
+  for (; t < ntiles; t += gridDim.x) {
+    const int64_t base = tile_base(A, t, K, MULTI);
+    if (A.src) {
+      stage_tile<R, TB>(tile, pre, tid);
+      __syncthreads();
+      const int64_t tn = t + gridDim.x;
+      if (tn < ntiles) fetch_tile<R, TB>(A, tile_base(A, tn, K, MULTI), g_tid, pre);
+    }
+    pass_body<R, TB, MULTI>(A, t, base, g_tid, tile, red);
+    __syncthreads();
+  }
 }
 
 // Work-item form: item i runs tile items[2i+1] of the pass described by
-// descs[items[2i]].  Circuits of different widths and programs share a launch.
+// descs[items[2i]].  Circuits of different widths and programs share a launch;
+// the next item's tile is fetched while the current one computes.
 template <int R, int TB>
-__global__ void __launch_bounds__(1 << TB)
+__global__ void __launch_bounds__(1 << TB) __attribute__((amdgpu_waves_per_eu(kPassWaves<R>)))
 k_pass_items(const PassArgs *descs, const int32_t *items, int nitems) {
   __shared__ __align__(16) float2 tile[1 << (R + TB)];
   __shared__ float red[((1 << TB) / 64) * (R + TB + 1)];
-  for (int i = blockIdx.x; i < nitems; i += gridDim.x) {
+  constexpr int K = R + TB;
+  const int tid = threadIdx.x;
+  float4 pre[1 << (R - 1)];
+  int i = blockIdx.x;
+  if (i < nitems) {
+    const PassArgs &A = descs[items[2 * i]];
+    if (A.src) fetch_tile<R, TB>(A, tile_base(A, items[2 * i + 1], K, A.multi), thread_offset<TB>(A, tid), pre);
+  }
+  for (; i < nitems; i += gridDim.x) {
     const PassArgs &A = descs[items[2 * i]];
     const int64_t t = items[2 * i + 1];
+    const int64_t base = tile_base(A, t, K, A.multi);
+    const int64_t g_tid = thread_offset<TB>(A, tid);
+    if (A.src) {
+      stage_tile<R, TB>(tile, pre, tid);
+      __syncthreads();
+    }
+    const int in = i + gridDim.x;
+    if (in < nitems) {
+      const PassArgs &An = descs[items[2 * in]];
+      if (An.src)
+        fetch_tile<R, TB>(An, tile_base(An, items[2 * in + 1], K, An.multi), thread_offset<TB>(An, tid), pre);
+    }
     if (A.multi)
-      pass_tile<R, TB, true>(A, t, tile, red);
+      pass_body<R, TB, true>(A, t, base, g_tid, tile, red);
     else
-      pass_tile<R, TB, false>(A, t, tile, red);
+      pass_body<R, TB, false>(A, t, base, g_tid, tile, red);
     __syncthreads();
   }
 }
@@ -485,10 +564,13 @@ k_compose(float *utab, float *qtab, const int32_t *words, const int32_t *off, in
 
 template <int R, int TB>
 static int launch_pass(const PassArgs &A, int64_t ntiles, hipStream_t st) {
-  if (A.multi)
-    hipLaunchKernelGGL((k_pass<R, TB, true>), dim3((unsigned)ntiles), dim3(1 << TB), 0, st, A);
-  else
-    hipLaunchKernelGGL((k_pass<R, TB, false>), dim3((unsigned)ntiles), dim3(1 << TB), 0, st, A);
+  if (A.multi) {
+    const unsigned g = resident_grid(k_pass<R, TB, true>, 1 << TB, ntiles);
+    hipLaunchKernelGGL((k_pass<R, TB, true>), dim3(g), dim3(1 << TB), 0, st, A, ntiles);
+  } else {
+    const unsigned g = resident_grid(k_pass<R, TB, false>, 1 << TB, ntiles);
+    hipLaunchKernelGGL((k_pass<R, TB, false>), dim3(g), dim3(1 << TB), 0, st, A, ntiles);
+  }
   return launch_status();
 }
 
@@ -528,8 +610,8 @@ static int fill_pass_args(PassArgs &A, float *state, const float *src, int64_t B
   const int K = R + TB;
   A.state = state;
   A.src = src;
-  A.cs = cs;
-  A.utab = utab;
+  A.cs = reinterpret_cast<const int32_t *>(cs);
+  A.utab = reinterpret_cast<const int32_t *>(utab);
   A.efac = efac;
   A.mpart = mpart;
   A.words = words;

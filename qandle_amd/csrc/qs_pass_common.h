@@ -28,13 +28,33 @@ __host__ __device__ __forceinline__ int swz(int l) {
 }
 
 // ---------------------------------------------------------------------------
-// v2 per-bit operations.  A 2x2 complex matrix is 8 floats
+// Per-bit operations.  A 2x2 complex matrix is 8 floats
 // (u00.re, u00.im, u01.re, u01.im, u10.re, u10.im, u11.re, u11.im), column
 // convention: (a0, a1) <- (u00 a0 + u01 a1, u10 a0 + u11 a1).
+//
+// The pass kernels hold their read-only tables (fused unitaries, gradient
+// generators, phase coefficients) as int32 pointers and convert the bits with
+// __int_as_float: no float store of a kernel can then alias a table read, so
+// the compiler keeps these uniform reads on the scalar (SMEM) path instead of
+// issuing vector loads that wait behind the tile's HBM traffic.
 // ---------------------------------------------------------------------------
+struct M8 {
+  float x[8];
+};
+
+__device__ __forceinline__ M8 load_m8(const int32_t *p) {
+  M8 m;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) m.x[i] = __int_as_float(p[i]);
+  return m;
+}
+
+// A fused unitary on the vector ALU (two complex multiply-adds per output);
+// used where the four lanes of a matrix-core block may hold different
+// unitaries (per-sample unitaries in a tile of several samples).
 template <int BIT, int NR>
-__device__ __forceinline__ void apply_u(float2 (&v)[NR], const float *u) {
-  const float a = u[0], b = u[1], c = u[2], d = u[3], e = u[4], f = u[5], g = u[6], h = u[7];
+__device__ __forceinline__ void apply_valu(float2 (&v)[NR], const M8 &u) {
+  const float a = u.x[0], b = u.x[1], c = u.x[2], d = u.x[3], e = u.x[4], f = u.x[5], g = u.x[6], h = u.x[7];
 #pragma unroll
   for (int j = 0; j < NR; ++j) {
     if (!(j & (1 << BIT))) {
@@ -45,16 +65,63 @@ __device__ __forceinline__ void apply_u(float2 (&v)[NR], const float *u) {
   }
 }
 
-// (a0, a1) <- U^dagger (a0, a1) = (conj(u00) a0 + conj(u10) a1, conj(u01) a0 + conj(u11) a1)
+__device__ __forceinline__ M8 dagger(const M8 &u) {
+  M8 t;
+  t.x[0] = u.x[0]; t.x[1] = -u.x[1]; t.x[2] = u.x[4]; t.x[3] = -u.x[5];
+  t.x[4] = u.x[2]; t.x[5] = -u.x[3]; t.x[6] = u.x[6]; t.x[7] = -u.x[7];
+  return t;
+}
+
+// A fused unitary runs on the matrix cores.  In the real form (re0, im0, re1,
+// im1) of an amplitude pair, U is the 4x4 matrix
+//   [[a, -b, c, -d], [b, a, d, c], [e, -f, g, -h], [f, e, h, g]]
+// (u00 = a + ib, u01 = c + id, u10 = e + if, u11 = g + ih) and U^dagger is its
+// transpose.  v_mfma_f32_4x4x1f32 computes, in 16 blocks of 4 lanes,
+// D[i][j] += A(lane 4b + i) * B(lane 4b + j) with output row i in accumulator
+// register i of lane 4b + j: with A_k(lane) = U[lane & 3][k] and B_k(lane) =
+// component k of the lane's own pair, four steps k = 0..3 give every lane
+// U times its pair.  The matrix units run at twice the FP32 rate of the
+// vector ALU and leave it free for the rest of the pass.
+typedef float qs_f4 __attribute__((ext_vector_type(4)));
+
+struct MU {
+  float a[4];  // A_k of this lane, k = 0..3
+};
+
+// Coefficients of U (or of U^dagger when DAG) for the lane's matrix row.
+template <bool DAG>
+__device__ __forceinline__ MU mfma_coeffs(const M8 &u) {
+  const float a = u.x[0], b = u.x[1], c = u.x[2], d = u.x[3], e = u.x[4], f = u.x[5], g = u.x[6], h = u.x[7];
+  const int row = threadIdx.x & 3;
+  const bool r1 = row & 1, r2 = row & 2;
+  MU m;
+  if (!DAG) {  // A_k = U[row][k]
+    m.a[0] = r2 ? (r1 ? f : e) : (r1 ? b : a);
+    m.a[1] = r2 ? (r1 ? e : -f) : (r1 ? a : -b);
+    m.a[2] = r2 ? (r1 ? h : g) : (r1 ? d : c);
+    m.a[3] = r2 ? (r1 ? g : -h) : (r1 ? c : -d);
+  } else {  // A_k = U[k][row]
+    m.a[0] = r2 ? (r1 ? -d : c) : (r1 ? -b : a);
+    m.a[1] = r2 ? (r1 ? c : d) : (r1 ? a : b);
+    m.a[2] = r2 ? (r1 ? -h : g) : (r1 ? -f : e);
+    m.a[3] = r2 ? (r1 ? g : h) : (r1 ? e : f);
+  }
+  return m;
+}
+
 template <int BIT, int NR>
-__device__ __forceinline__ void apply_udag(float2 (&v)[NR], const float *u) {
-  const float a = u[0], b = -u[1], c = u[4], d = -u[5], e = u[2], f = -u[3], g = u[6], h = -u[7];
+__device__ __forceinline__ void apply_mfma(float2 (&v)[NR], const MU &m) {
 #pragma unroll
   for (int j = 0; j < NR; ++j) {
     if (!(j & (1 << BIT))) {
       const float2 x = v[j], y = v[j | (1 << BIT)];
-      v[j] = make_float2(a * x.x - b * x.y + c * y.x - d * y.y, a * x.y + b * x.x + c * y.y + d * y.x);
-      v[j | (1 << BIT)] = make_float2(e * x.x - f * x.y + g * y.x - h * y.y, e * x.y + f * x.x + g * y.y + h * y.x);
+      qs_f4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = __builtin_amdgcn_mfma_f32_4x4x1f32(m.a[0], x.x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_4x4x1f32(m.a[1], x.y, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_4x4x1f32(m.a[2], y.x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_4x4x1f32(m.a[3], y.y, acc, 0, 0, 0);
+      v[j] = make_float2(acc[0], acc[1]);
+      v[j | (1 << BIT)] = make_float2(acc[2], acc[3]);
     }
   }
 }
@@ -62,7 +129,7 @@ __device__ __forceinline__ void apply_udag(float2 (&v)[NR], const float *u) {
 // Cross density of bit BIT over the thread's registers: M_ab = sum psi_a conj(lam_b),
 // 8 floats (m00.re, m00.im, m01.re, m01.im, m10.re, m10.im, m11.re, m11.im).
 template <int BIT, int NR>
-__device__ __forceinline__ void cross_density(const float2 (&p)[NR], const float2 (&l)[NR], float *m) {
+__device__ __forceinline__ M8 cross_density(const float2 (&p)[NR], const float2 (&l)[NR]) {
   float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f, m6 = 0.f, m7 = 0.f;
 #pragma unroll
   for (int j = 0; j < NR; ++j) {
@@ -75,57 +142,59 @@ __device__ __forceinline__ void cross_density(const float2 (&p)[NR], const float
       m6 += p1.x * l1.x + p1.y * l1.y;  m7 += p1.y * l1.x - p1.x * l1.y;
     }
   }
-  m[0] = m0; m[1] = m1; m[2] = m2; m[3] = m3; m[4] = m4; m[5] = m5; m[6] = m6; m[7] = m7;
+  return M8{{m0, m1, m2, m3, m4, m5, m6, m7}};
 }
 
 // run-time bit -> compile-time template (the switch folds away once the
 // caller's loop over register bits is unrolled)
 template <int R>
-__device__ __forceinline__ void apply_u_at(float2 (&v)[1 << R], int r, const float *u) {
+__device__ __forceinline__ void apply_valu_at(float2 (&v)[1 << R], int r, const M8 &u) {
   switch (r) {
-    case 0: apply_u<0>(v, u); break;
-    case 1: if constexpr (R > 1) apply_u<1>(v, u); break;
-    case 2: if constexpr (R > 2) apply_u<2>(v, u); break;
-    case 3: if constexpr (R > 3) apply_u<3>(v, u); break;
-    case 4: if constexpr (R > 4) apply_u<4>(v, u); break;
+    case 0: apply_valu<0>(v, u); break;
+    case 1: if constexpr (R > 1) apply_valu<1>(v, u); break;
+    case 2: if constexpr (R > 2) apply_valu<2>(v, u); break;
+    case 3: if constexpr (R > 3) apply_valu<3>(v, u); break;
+    case 4: if constexpr (R > 4) apply_valu<4>(v, u); break;
     default: break;
   }
 }
 template <int R>
-__device__ __forceinline__ void apply_udag_at(float2 (&v)[1 << R], int r, const float *u) {
+__device__ __forceinline__ void apply_mfma_at(float2 (&v)[1 << R], int r, const MU &m) {
   switch (r) {
-    case 0: apply_udag<0>(v, u); break;
-    case 1: if constexpr (R > 1) apply_udag<1>(v, u); break;
-    case 2: if constexpr (R > 2) apply_udag<2>(v, u); break;
-    case 3: if constexpr (R > 3) apply_udag<3>(v, u); break;
-    case 4: if constexpr (R > 4) apply_udag<4>(v, u); break;
+    case 0: apply_mfma<0>(v, m); break;
+    case 1: if constexpr (R > 1) apply_mfma<1>(v, m); break;
+    case 2: if constexpr (R > 2) apply_mfma<2>(v, m); break;
+    case 3: if constexpr (R > 3) apply_mfma<3>(v, m); break;
+    case 4: if constexpr (R > 4) apply_mfma<4>(v, m); break;
     default: break;
   }
 }
 template <int R>
-__device__ __forceinline__ void cross_density_at(const float2 (&p)[1 << R], const float2 (&l)[1 << R], int r, float *m) {
+__device__ __forceinline__ M8 cross_density_at(const float2 (&p)[1 << R], const float2 (&l)[1 << R], int r) {
   switch (r) {
-    case 0: cross_density<0>(p, l, m); break;
-    case 1: if constexpr (R > 1) cross_density<1>(p, l, m); break;
-    case 2: if constexpr (R > 2) cross_density<2>(p, l, m); break;
-    case 3: if constexpr (R > 3) cross_density<3>(p, l, m); break;
-    case 4: if constexpr (R > 4) cross_density<4>(p, l, m); break;
+    case 1: if constexpr (R > 1) return cross_density<1>(p, l); break;
+    case 2: if constexpr (R > 2) return cross_density<2>(p, l); break;
+    case 3: if constexpr (R > 3) return cross_density<3>(p, l); break;
+    case 4: if constexpr (R > 4) return cross_density<4>(p, l); break;
     default: break;
   }
+  return cross_density<0>(p, l);
 }
 
 // Per-thread phase of a sub-pass's RZ gates on non-register bits:
-// prod_k e^{-/+ i phi_k/2} by the thread's value of bit pos_k.
-__device__ __forceinline__ float2 phase_product(const int32_t *ph, int nphase, const float2 *cs, int64_t csample,
+// prod_k e^{-/+ i phi_k/2} by the thread's value of bit pos_k.  cs holds the
+// (cos, sin) float pairs as int32 bits.
+__device__ __forceinline__ float2 phase_product(const int32_t *ph, int nphase, const int32_t *cs, int64_t csample,
                                                 int64_t gbase, bool inverse) {
   float pr = 1.f, pi = 0.f;
   for (int k = 0; k < nphase; ++k) {
     const int fl = ph[4 * k + 2];
     if (fl & QS_SEG_NOUNAPPLY) continue;
-    const float2 c_s = cs[ph[4 * k + 1] + ((fl & QS_SEG_PER_SAMPLE) ? csample : 0)];
-    float sn = ((gbase >> ph[4 * k]) & 1) ? -c_s.y : c_s.y;
+    const int64_t idx = ph[4 * k + 1] + ((fl & QS_SEG_PER_SAMPLE) ? csample : 0);
+    const float c = __int_as_float(cs[2 * idx]), s = __int_as_float(cs[2 * idx + 1]);
+    float sn = ((gbase >> ph[4 * k]) & 1) ? -s : s;
     if (inverse) sn = -sn;
-    const float nr = pr * c_s.x + pi * sn, ni = pi * c_s.x - pr * sn;  // (pr + i pi)(c - i sn)
+    const float nr = pr * c + pi * sn, ni = pi * c - pr * sn;  // (pr + i pi)(c - i sn)
     pr = nr;
     pi = ni;
   }
@@ -133,29 +202,10 @@ __device__ __forceinline__ float2 phase_product(const int32_t *ph, int nphase, c
 }
 
 // Im Tr(Q M) = Im sum_ab Q_ab M_ba
-__device__ __forceinline__ float im_trace(const float *q, const float *m) {
+__device__ __forceinline__ float im_trace(const M8 &q, const M8 &m) {
   // Q00 M00 + Q01 M10 + Q10 M01 + Q11 M11, imaginary parts
-  return (q[0] * m[1] + q[1] * m[0]) + (q[2] * m[5] + q[3] * m[4]) +
-         (q[4] * m[3] + q[5] * m[2]) + (q[6] * m[7] + q[7] * m[6]);
-}
-
-template <int KIND, int BIT, int NR>
-__device__ __forceinline__ void rot_reg(float2 (&v)[NR], float c, float s) {
-#pragma unroll
-  for (int j = 0; j < NR; ++j)
-    if (!(j & (1 << BIT))) rot_pair<KIND>(c, s, v[j], v[j | (1 << BIT)]);
-}
-
-template <int KIND, int R>
-__device__ __forceinline__ void rot_dispatch(float2 (&v)[1 << R], int bit, float c, float s) {
-  switch (bit) {
-    case 0: rot_reg<KIND, 0>(v, c, s); break;
-    case 1: if constexpr (R > 1) rot_reg<KIND, 1>(v, c, s); break;
-    case 2: if constexpr (R > 2) rot_reg<KIND, 2>(v, c, s); break;
-    case 3: if constexpr (R > 3) rot_reg<KIND, 3>(v, c, s); break;
-    case 4: if constexpr (R > 4) rot_reg<KIND, 4>(v, c, s); break;
-    default: break;
-  }
+  return (q.x[0] * m.x[1] + q.x[1] * m.x[0]) + (q.x[2] * m.x[5] + q.x[3] * m.x[4]) +
+         (q.x[4] * m.x[3] + q.x[5] * m.x[2]) + (q.x[6] * m.x[7] + q.x[7] * m.x[6]);
 }
 
 template <int CB, int TBIT, int NR>
@@ -211,41 +261,16 @@ __device__ __forceinline__ void swap_dispatch(float2 (&v)[1 << R], int tb) {
   }
 }
 
-// Applies the ops of one sub-pass to the thread's registers.
+// One CNOT word: bit 0 = control outside the registers (its value is uniform
+// per thread), bits 8-15 = target register, bits 16+ = control (register
+// index, or global bit position when uniform).
 template <int R>
-__device__ __forceinline__ void run_ops(float2 (&v)[1 << R], const int32_t *op, int nops,
-                                        const float2 *cs, int64_t sample, int64_t gbase) {
-  for (int o = 0; o < nops; ++o, op += 4) {
-    const int code = op[0];
-    const int a = op[1], b = op[2];
-    const int kind = code & 0xff;
-    if (kind == QS_OP_CNOT) {
-      cnot_dispatch<R>(v, b, a);
-      continue;
-    }
-    if (kind == QS_OP_CNOT_UNIFORM) {
-      if ((gbase >> b) & 1) swap_dispatch<R>(v, a);
-      continue;
-    }
-    const int64_t idx = op[3] + ((code & QS_OP_PER_SAMPLE) ? sample : 0);
-    const float2 c_s = cs[idx];
-    if (kind == QS_OP_RX) {
-      rot_dispatch<QS_RX, R>(v, a, c_s.x, c_s.y);
-    } else if (kind == QS_OP_RY) {
-      rot_dispatch<QS_RY, R>(v, a, c_s.x, c_s.y);
-    } else if (kind == QS_OP_RZ) {
-      rot_dispatch<QS_RZ, R>(v, a, c_s.x, c_s.y);
-    } else {  // QS_OP_RZ_UNIFORM: phase e^{-i phi/2} on bit 0, e^{+i phi/2} on bit 1
-      const float c = c_s.x;
-      const float s = ((gbase >> a) & 1) ? -c_s.y : c_s.y;
-#pragma unroll
-      for (int j = 0; j < (1 << R); ++j) {
-        float2 x = v[j];
-        v[j] = make_float2(c * x.x + s * x.y, c * x.y - s * x.x);
-      }
-    }
-  }
+__device__ __forceinline__ void run_cnot(float2 (&v)[1 << R], int cw, int64_t gbase) {
+  const int tgt = (cw >> 8) & 0xff, ctl = cw >> 16;
+  if (!(cw & 1))
+    cnot_dispatch<R>(v, ctl, tgt);
+  else if ((gbase >> ctl) & 1)
+    swap_dispatch<R>(v, tgt);
 }
-
 
 }  // namespace qs

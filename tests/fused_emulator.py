@@ -65,37 +65,61 @@ def compose(recipes_words, B, cs, nu, nq):
     return utab, qtab
 
 
-def _subpasses(words, cnots_first=False):
-    """Walks the words with one cursor in the order the kernel reads them: the
-    forward kernel reads a segment's unitary entries, then its CNOTs; the
-    adjoint kernel (``cnots_first``) the CNOTs first."""
+def _cnot_fields(cw):
+    """(uniform control?, target register, control register or global position)."""
+    return bool(cw & PL.CNOT_UNIFORM), (cw >> 8) & 0xFF, cw >> 16
+
+
+def _subpasses(words, R, adjoint=False):
+    """Walks the words with one cursor in the order the kernel reads them.
+
+    Returns per sub-pass (header, phases, segments); a segment is (bits, cnots)
+    with bits = [(r, utab index, per-sample?, probe?, nconst, qtab index, rows)]
+    for the masked register bits and cnots = [(uniform?, target, control)].
+    The forward kernel reads a fixed record (head + R utab indices) then the
+    CNOT words; the adjoint kernel the head, the CNOT words, then one variable
+    entry per masked bit."""
     pos, out = 0, []
     while pos < len(words):
         hdr = words[pos: pos + PL.HDR]
         nseg, nph = int(hdr[0]), int(hdr[6])
+        assert nseg >= 0 and nph >= 0
         p = pos + PL.HDR + 4 * nph
         segs = []
         for _ in range(nseg):
             head = int(words[p])
             p += 1
-            umask, ncnot = head & 0xFF, head >> 8
-            if cnots_first:
-                cn = [tuple(int(x) for x in words[p + 4 * c: p + 4 * c + 4]) for c in range(ncnot)]
-                p += 4 * ncnot
+            umask, ncnot = head & 0xFF, (head >> 8) & 0xFF
+            psmask, probemask = (head >> 16) & 0xFF, (head >> 24) & 0x7F
+            assert umask < (1 << R) and (psmask | probemask) & ~umask == 0
             bits = []
-            for r in range(8):
-                if (umask >> r) & 1:
-                    ui, fl, nc, qi = (int(x) for x in words[p: p + 4])
-                    rows = [int(x) for x in words[p + 4: p + 4 + nc]]
-                    bits.append((r, ui, fl, nc, qi, rows))
-                    p += 4 + nc
-            if not cnots_first:
-                cn = [tuple(int(x) for x in words[p + 4 * c: p + 4 * c + 4]) for c in range(ncnot)]
-                p += 4 * ncnot
+            if adjoint:
+                cn = [_cnot_fields(int(x)) for x in words[p: p + ncnot]]
+                p += ncnot
+                for r in range(R):
+                    if (umask >> r) & 1:
+                        ui, nc, qi = (int(x) for x in words[p: p + 3])
+                        rows = [int(x) for x in words[p + 3: p + 3 + nc]]
+                        assert len(rows) == nc
+                        bits.append((r, ui, bool((psmask >> r) & 1), bool((probemask >> r) & 1), nc, qi, rows))
+                        p += 3 + nc
+            else:
+                idx = [int(x) for x in words[p: p + R]]
+                assert len(idx) == R
+                p += R
+                for r in range(R):
+                    if (umask >> r) & 1:
+                        bits.append((r, idx[r], bool((psmask >> r) & 1), False, 0, 0, []))
+                    else:
+                        assert idx[r] == 0
+                cn = [_cnot_fields(int(x)) for x in words[p: p + ncnot]]
+                p += ncnot
+            assert len(cn) == ncnot
             segs.append((bits, cn))
         phases = [tuple(int(x) for x in words[pos + PL.HDR + 4 * k: pos + PL.HDR + 4 * k + 4]) for k in range(nph)]
         out.append((hdr, phases, segs))
         pos = p
+    assert pos == len(words), "the walk must end exactly at the end of the words"
     return out
 
 
@@ -189,7 +213,7 @@ def emulate_pass(state, src, B, n, R, TB, lpos, words, cs, utab, efac, measure):
     spt_log = (K - n) if g.multi else 0
     mparts = np.zeros((g.ntiles << spt_log, K + 1))
     swz = np.vectorize(PL.swz)
-    subs = _subpasses(words)
+    subs = _subpasses(words, R)
     for t in range(g.ntiles):
         base = g.base(t, n)
         tile = np.zeros(1 << K, dtype=np.complex128)
@@ -220,13 +244,13 @@ def emulate_pass(state, src, B, n, R, TB, lpos, words, cs, utab, efac, measure):
             else:
                 v = tile[addr].copy()
             for bits, cn in segs:
-                for r, ui, fl, nc, qi, rows in bits:
-                    M = utab[ui + (csample if fl & PL.SEG_PER_SAMPLE else 0)]
+                for r, ui, ps, _probe, _nc, _qi, _rows in bits:
+                    M = utab[ui + (csample if ps else 0)]
                     if M.ndim == 2:
                         M = np.broadcast_to(M, (T, 2, 2))
                     _apply_pair(v, r, M)
-                for code, a, b, _ in cn:
-                    if code == PL.OP_CNOT:
+                for uniform, a, b in cn:
+                    if not uniform:
                         _cnot(v, a, b)
                     else:
                         _swap_where(v, a, ((gbase >> b) & 1).astype(bool))
@@ -255,7 +279,7 @@ def emulate_adjoint_pass(psi, lam, B, n, R, TB, lpos, words, cs, utab, qtab, gra
     g = _Geom(B, n, R, TB, lpos)
     K, T, NR = g.K, g.T, g.NR
     swz = np.vectorize(PL.swz)
-    subs = _subpasses(words, cnots_first=True)
+    subs = _subpasses(words, R, adjoint=True)
     for t in range(g.ntiles):
         base = g.base(t, n)
         ls = np.arange(1 << K, dtype=np.int64)
@@ -295,16 +319,15 @@ def emulate_adjoint_pass(psi, lam, B, n, R, TB, lpos, words, cs, utab, qtab, gra
                 vp *= z[:, None]
                 vl *= z[:, None]
             for bits, cn in segs:
-                for code, a, b, _ in cn:
-                    if code == PL.OP_CNOT:
+                for uniform, a, b in cn:
+                    if not uniform:
                         _cnot(vp, a, b)
                         _cnot(vl, a, b)
                     else:
                         cond = ((gbase >> b) & 1).astype(bool)
                         _swap_where(vp, a, cond)
                         _swap_where(vl, a, cond)
-                for r, ui, fl, nc, qi, rows in bits:
-                    ps = fl & PL.SEG_PER_SAMPLE
+                for r, ui, ps, probe, nc, qi, rows in bits:
                     span = B if ps else 1
                     var = csample if ps else np.zeros(T, dtype=np.int64)
                     M = np.zeros((T, 2, 2), complex)
@@ -318,7 +341,7 @@ def emulate_adjoint_pass(psi, lam, B, n, R, TB, lpos, words, cs, utab, qtab, gra
                     for k in range(nc):
                         Q = qtab[qi + k * span + var]
                         put(np.imag(np.einsum("tab,tba->t", Q, M)), rows[k])
-                    if not fl & PL.SEG_NOUNAPPLY:
+                    if not probe:
                         U = utab[ui + var]
                         Ud = np.conj(np.swapaxes(U, -1, -2))
                         _apply_pair(vp, r, Ud)
@@ -384,7 +407,7 @@ def check_words(words, R, TB, n, B, ncs, nu, nq, nrows, adjoint, nsub=None):
     """Walks one pass's words with the kernel's cursor and asserts that every
     index the kernel derives from them stays inside its table or register file."""
     K = R + TB
-    subs = _subpasses(words, cnots_first=adjoint)  # consumes every word or raises
+    subs = _subpasses(words, R, adjoint)  # consumes exactly every word or raises
     assert nsub is None or len(subs) == nsub
     posmax = max(n, K)  # a multi-sample tile addresses sample bits above n - 1
     for hdr, phases, segs in subs:
@@ -401,17 +424,16 @@ def check_words(words, R, TB, n, B, ncs, nu, nq, nrows, adjoint, nsub=None):
                 assert 0 <= ci and ci + ((B - 1) if fl & PL.SEG_PER_SAMPLE else 0) < ncs
             assert row == -1 or 0 <= row < nrows
         for bits, cn in segs:
-            for r, ui, fl, nc, qi, rows in bits:
-                span = B if fl & PL.SEG_PER_SAMPLE else 1
-                assert 0 <= r < R and nc >= 1
+            for r, ui, ps, probe, nc, qi, rows in bits:
+                span = B if ps else 1
+                assert 0 <= r < R
                 assert 0 <= ui and ui + span <= nu
                 if adjoint:
-                    assert 0 <= qi and qi + nc * span <= nq
+                    assert nc >= 1 and 0 <= qi and qi + nc * span <= nq
                     assert all(-1 <= x < nrows for x in rows)
-            for code, a, b, _ in cn:
-                assert code in (PL.OP_CNOT, PL.OP_CNOT_UNIFORM)
+            for uniform, a, b in cn:
                 assert 0 <= a < R
-                assert (0 <= b < R and b != a) if code == PL.OP_CNOT else (0 <= b < n)
+                assert (0 <= b < n) if uniform else (0 <= b < R and b != a)
 
 
 def check_recipes(recipes, offsets, ncs, nu, nq, generators):
@@ -432,7 +454,7 @@ def emulate_program(prog_gates, n, B, R, TB, cs, efac, src=None, measure=True, c
     """Plan + encode + compose + emulate the forward; returns (state, probs or None, plan)."""
     pl = PL.plan(n, prog_gates, R, TB, cm)
     tb = PL.TableBuilder(B)
-    words = [PL.encode_pass_v2(p, prog_gates, R, TB, lambda s: s, lambda s: False, tb) for p in pl.passes]
+    words = [PL.encode_pass(p, prog_gates, R, TB, lambda s: s, lambda s: False, tb) for p in pl.passes]
     utab, _ = compose(tb.recipes(), B, cs, tb.nu, tb.nq)
     state, probs = run_forward_words(list(zip(words, [p.L for p in pl.passes])), B, n, R, TB, cs, utab, efac,
                                      src=src, measure=measure)
@@ -449,14 +471,14 @@ def emulate_adjoint(prog_gates, n, B, R, TB, cs, psi_final, w, pl, embed_bits=No
             rows[gi] = len(rows)
     nrot = len(rows)
     tb = PL.TableBuilder(B)
-    words = [PL.encode_pass_v2(p, prog_gates, R, TB, lambda s: s, lambda s: False, tb, reverse=True,
+    words = [PL.encode_pass(p, prog_gates, R, TB, lambda s: s, lambda s: False, tb, reverse=True,
                                row_of=rows.__getitem__) for p in reversed(pl.passes)]
     lpos = [p.L for p in reversed(pl.passes)]
     if embed_bits:
         ggates = [PL.Gate(PL.PGRAD + embed_rot, b, -1, j) for j, b in enumerate(embed_bits)]
         gplan = PL.plan(n, ggates, R, TB)
         for p in gplan.passes:
-            words.append(PL.encode_pass_v2(p, ggates, R, TB, lambda s: 0, lambda s: False, tb, reverse=True,
+            words.append(PL.encode_pass(p, ggates, R, TB, lambda s: 0, lambda s: False, tb, reverse=True,
                                            row_of=lambda gi: nrot + gi))
             lpos.append(p.L)
     utab, qtab = compose(tb.recipes(), B, cs, tb.nu, tb.nq)
