@@ -29,6 +29,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 CPU_THREADS = 16
+_T0 = time.perf_counter()
+
+
+def log(msg: str) -> None:
+    """Progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
 def parse():
@@ -128,6 +134,7 @@ def main():
         return e0.elapsed_time(e1) / 1e3, wall
 
     # ---------------------------------------------------------------- headline
+    log("headline: 20 qubits, batch 64, 10 layers (first call compiles the pass kernels)")
     n, B, depth = 20, 64, 10
     w = workloads.rxryrz_ladder(n=n, B=B, depth=depth, seed=1)
     circ = qandle.Circuit(w.layers, num_qubits=n).to(dev)
@@ -141,6 +148,7 @@ def main():
 
     gpu_s, wall_s = time_steps(step, args.steps, args.warmup)
     ms_per_step = gpu_s / args.steps * 1e3
+    log(f"headline: {ms_per_step:.2f} ms per step")
     updates = w.amplitude_updates()
     value = updates * args.steps / gpu_s
 
@@ -186,7 +194,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"k_pass<{R},{TB},false> (qs_fused_pass)",
+            "kernel": (f"qs_pass_<i> (pass kernels specialised per pass program, hipRTC; R={R}, TB={TB})"
+                       if fused.use_jit(n, R, TB) else f"k_pass<{R},{TB},false> (qs_fused_pass)"),
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -202,6 +211,8 @@ def main():
     if not args.quick:
         extras = {}
         # plain per-gate GPU path on the same workload
+        log("plain per-gate path on the headline workload")
+
         def plain_step():
             with torch.no_grad():
                 return circ.circuit.forward_layers(x=x)
@@ -214,6 +225,7 @@ def main():
         extras["plain_step_hbm_frac"] = (B * 2 ** n * (8 + 16 * len(w.gates) + 8)) / (p_s / 2) / 1e9 / HBM_PEAK_GBS
 
         # 24 qubits, batch 4 (same circuit family)
+        log("24 qubits, batch 4")
         w24 = workloads.rxryrz_ladder(n=24, B=4, depth=10, seed=1)
         c24 = qandle.Circuit(w24.layers, num_qubits=24).to(dev)
         x24 = w24.inputs["x"].to(dev)
@@ -228,6 +240,7 @@ def main():
         extras["n24_passes"] = np24
 
         # adjoint forward + backward, 16 qubits, batch 256, 20 layers
+        log("adjoint forward + backward: 16 qubits, batch 256, 20 layers")
         w3 = workloads.rxryrz_ladder(n=16, B=256, depth=20, seed=2, loss=True)
         c3 = qandle.Circuit(w3.layers, num_qubits=16).to(dev)
         x3 = w3.inputs["x"].to(dev).requires_grad_(True)
@@ -244,6 +257,7 @@ def main():
             (c3.circuit.forward_layers(x=x3) * lw).sum().backward()
 
         ta, _ = time_steps(adj_step, 3, 1)
+        log(f"adjoint: {ta / 3 * 1e3:.2f} ms per forward + backward; plain per-gate autograd next")
         tp, _ = time_steps(plain_adj_step, 1, 1)
         extras["adjoint_fwd_bwd_ms"] = ta / 3 * 1e3
         extras["plain_autograd_fwd_bwd_ms"] = tp * 1e3
@@ -252,15 +266,19 @@ def main():
         torch.cuda.empty_cache()
 
         # uneven batch: 2048 circuits, 4..18 qubits, depth 2..24, forward + adjoint in one graph
+        log("uneven batch of 2048 circuits")
         extras.update(bench_uneven(dev, time_steps))
         torch.cuda.empty_cache()
         # serving: Poisson arrivals at ~70% offered load of the measured fused throughput
+        log("serving loop")
         extras.update(bench_serving(dev, value))
         torch.cuda.empty_cache()
         result["extras"] = extras
 
         if not args.no_cpu:
+            log("CPU baseline sample")
             result["cpu_baseline"] = cpu_baseline(w)
+    log("done")
     print(json.dumps(result))
 
 

@@ -103,16 +103,24 @@ class FusedCircuitFn(torch.autograd.Function):
         wb = lib.qs_adjoint_work_bytes(B, n, R, TB, total_rows)
         gpart = torch.empty(max(wb // 4, 1), dtype=torch.float32, device=psi.device)
         npass = len(pl.passes)
+        kern = None
+        if fused.use_jit(n, R, TB):
+            kern = specialised_adjoint(prog, pl, ws, B, embed, seed is not None)
+            ntiles, NS = adjoint_geometry(B, n, R, TB)
         for k, (w, nsub, lpos) in enumerate(ws.passes):
-            L.check(
-                lib.qs_adjoint_pass(
-                    L.ptr(psi), L.ptr(lam), B, n, R, TB,
-                    lpos.ctypes.data_as(L.ctypes.POINTER(L.ctypes.c_int32)), L.ptr(w), nsub,
-                    L.ptr(cs), L.ptr(utab), L.ptr(qtab), L.ptr(gpart), total_rows,
-                    None if (k > 0 or seed is None) else L.ptr(seed), st,
-                ),
-                "qs_adjoint_pass",
-            )
+            seed_ptr = None if (k > 0 or seed is None) else L.ptr(seed)
+            if kern is not None:
+                kern.launch(k, L.ptr(psi), L.ptr(lam), L.ptr(utab), L.ptr(qtab), L.ptr(cs), L.ptr(gpart), seed_ptr,
+                            B, NS, ntiles, st)
+            else:
+                L.check(
+                    lib.qs_adjoint_pass(
+                        L.ptr(psi), L.ptr(lam), B, n, R, TB,
+                        lpos.ctypes.data_as(L.ctypes.POINTER(L.ctypes.c_int32)), L.ptr(w), nsub,
+                        L.ptr(cs), L.ptr(utab), L.ptr(qtab), L.ptr(gpart), total_rows, seed_ptr, st,
+                    ),
+                    "qs_adjoint_pass",
+                )
             i = npass - 1 - k  # forward pass this step undid
             if k < nmain and (i - 1) in ckpts:
                 psi.copy_(ckpts[i - 1])
@@ -155,6 +163,30 @@ class FusedCircuitFn(torch.autograd.Function):
             shp = ctx.state_shape
             g_state = lam.sum(0) if len(shp) == 1 else (lam if lam.shape == shp else lam.sum(0, keepdim=True))
         return (None, None, None, None, None, None, None, g_param, g_embed, g_state, *g_named)
+
+
+def adjoint_geometry(B: int, n: int, R: int, TB: int):
+    """(tiles, reduction slots per gradient row) of an adjoint pass, as the
+    library's adjoint kernels and qs_adjoint_finish lay them out."""
+    K = R + TB
+    multi = n < K
+    seglog = (n - R) if multi else TB
+    wlog = min(seglog, 6)
+    ntiles = ((B + (1 << (K - n)) - 1) >> (K - n)) if multi else (B << (n - K))
+    return ntiles, ntiles * (1 << (TB - wlog))
+
+
+def specialised_adjoint(prog, pl, ws, B, embed, seeded):
+    """Run-time compiled kernels of this plan's adjoint sweep (cached on the program)."""
+    from . import jit
+
+    opts = getattr(config, "SPECIALISE_ADJOINT_OPTIONS", {}) or {}
+    key = ("jit_adj", id(pl), B, embed, seeded, tuple(sorted(opts.items())))
+    hit = prog.words_cache.get(key)
+    if hit is None:
+        hit = jit.AdjointKernels(ws.passes, prog.n, pl.R, pl.TB, seeded, **opts)
+        prog.words_cache[key] = hit
+    return hit
 
 
 def run_with_grad(prog, inp, R, TB, cm=None):

@@ -152,7 +152,7 @@ def get_plan(prog: Program, R: int, TB: int, cm: PL.CostModel = None) -> PL.Plan
     return pl
 
 
-DEFAULT_COST = PL.CostModel(budget=160.0, min_low=4)
+DEFAULT_COST = PL.CostModel(budget=256.0, min_low=4)
 
 
 def choose_config(prog: Program, inp):
@@ -172,9 +172,13 @@ def choose_config(prog: Program, inp):
 
 
 def choose_shape(n: int):
-    """(R, TB) for a circuit width: 32 amplitudes per thread and one wave per
-    tile (11 local bits) where the width allows, else every bit in registers
-    of a 256-thread tile that holds several samples."""
+    """(R, TB) for a circuit width: from 12 qubits, tiles of 2^12 amplitudes
+    over 256 threads with 16 amplitudes per thread (the widest tile whose
+    passes and adjoint sweeps keep two or more waves per SIMD); from 5 qubits,
+    2^11-amplitude tiles of one wave with 32 amplitudes per thread; below
+    that, every bit in registers of a 256-thread tile holding several samples."""
+    if n >= 12:
+        return 4, 8
     if n >= 5:
         return 5, 6
     return n, DEFAULT_TB
@@ -264,14 +268,66 @@ def gather_inputs(prog: Program, circuit, state, kwargs) -> Inputs:
                 named[name] = v.expand(B)
             else:
                 raise RuntimeError(f"named input '{name}' has {v.shape[0]} entries, batch is {B}")
+    return Inputs(B, batched, device, param_angles(prog, device), named, embed_x, state)
+
+
+def _first_angle(theta: torch.Tensor) -> torch.Tensor:
+    return theta.reshape(1) if theta.numel() == 1 else theta.reshape(-1)[:1]
+
+
+def param_angles(prog: Program, device) -> torch.Tensor:
+    """Remapped angles of the program's unnamed rotations, in slot order.
+
+    Each gate's angle is remapping(theta) as in the reference
+    (src/qandle/operators.py:265-275).  The qw_map remappings are elementwise,
+    so the thetas that share one are stacked and remapped in one launch (and
+    their gradients come back through one stack) instead of one launch per
+    gate; any other remapping callable is applied per gate.  Fixed angles are
+    uploaded once per program and device."""
+    from .. import qw_map
+
     params = [prog.slots[i] for i in prog.param_slots]
-    if params:
-        phis = [(s.module.remapping(s.module.theta).reshape(-1)[:1] if s.kind == "param"
-                 else torch.tensor([s.value])).to(device=device, dtype=torch.float32) for s in params]
-        phi_param = torch.cat(phis)
-    else:
-        phi_param = torch.zeros(0, dtype=torch.float32, device=device)
-    return Inputs(B, batched, device, phi_param, named, embed_x, state)
+    if not params:
+        return torch.zeros(0, dtype=torch.float32, device=device)
+    elementwise = (qw_map.none, qw_map.tanh)
+    pieces, order = [], []
+    groups: typing.Dict[typing.Any, list] = {}
+    values = []
+    for k, sl in enumerate(params):
+        if sl.kind == "param" and sl.module.remapping in elementwise:
+            groups.setdefault(sl.module.remapping, []).append(k)
+        elif sl.kind == "param":
+            pieces.append(sl.module.remapping(sl.module.theta).reshape(-1)[:1].to(device=device, dtype=torch.float32))
+            order.append([k])
+        else:
+            values.append(k)
+    for fn, ks in groups.items():
+        # a one-element theta enters through a reshape view (its gradient comes
+        # back as a slice of one buffer, with no per-parameter kernel)
+        th = torch.cat([_first_angle(params[k].module.theta) for k in ks])
+        pieces.append(fn(th).to(device=device, dtype=torch.float32))
+        order.append(ks)
+    if values:
+        key = ("values", str(device))
+        vt = prog.words_cache.get(key)
+        if vt is None:
+            vt = torch.tensor([params[k].value for k in values], dtype=torch.float32).to(device)
+            prog.words_cache[key] = vt
+        pieces.append(vt)
+        order.append(values)
+    flat = [k for ks in order for k in ks]
+    phi = pieces[0] if len(pieces) == 1 else torch.cat(pieces)
+    if flat == list(range(len(params))):
+        return phi
+    key = ("perm", str(device), tuple(flat))
+    inv = prog.words_cache.get(key)
+    if inv is None:
+        pos = [0] * len(flat)
+        for i, k in enumerate(flat):
+            pos[k] = i
+        inv = torch.tensor(pos, dtype=torch.long, device=device)
+        prog.words_cache[key] = inv
+    return phi.index_select(0, inv)
 
 
 def coefficient_layout(prog: Program, B: int):
@@ -458,11 +514,14 @@ def specialised_kernels(prog: Program, pl: PL.Plan, ws: WordSet, B: int, gen_fir
 
 
 def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
-            cm: PL.CostModel = None, want_state: bool = True, checkpoints=None, events=None):
+            cm: PL.CostModel = None, want_state: bool = True, checkpoints=None, events=None,
+            specialise: bool = None):
     """Runs every pass.  Returns (state or None, probs or None).
 
     ``events``: optional list; a (start, end) pair of HIP events recorded on the
-    launch stream around every pass kernel is appended to it."""
+    launch stream around every pass kernel is appended to it.  ``specialise``:
+    run compiled-per-program kernels (None: config.SPECIALISE) or the library's
+    interpreting kernels (False; no compilation on first use of a program)."""
     n, B = prog.n, inp.B
     pl = get_plan(prog, R, TB, cm)
     ws = pass_words(prog, pl, B, inp.device)
@@ -484,7 +543,8 @@ def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
         wb = lib.qs_fused_measure_work_bytes(B, n, R, TB)
         mpart = torch.empty(max(wb // 4, 1), dtype=torch.float32, device=inp.device)
     npass = len(words)
-    kern = specialised_kernels(prog, pl, ws, B, src is None) if use_jit(n, R, TB) else None
+    jit_on = use_jit(n, R, TB) if specialise is None else specialise
+    kern = specialised_kernels(prog, pl, ws, B, src is None) if jit_on else None
     ntiles = tiles_of(B, n, R + TB)
     for i, (w, nsub, lpos) in enumerate(words):
         last = i == npass - 1

@@ -16,6 +16,9 @@ source of one kernel with everything constant:
 * the RZ phases, the product-state generator and the MeasureProbability
   partials with their positions fixed.
 
+The kernels of one circuit's passes form one program: hipRTC compiles one
+program at a time, and the compiler start-up is paid once per program.
+
 The kernel keeps the interpreted kernel's tile pipeline (the next tile's HBM
 loads in flight during the current tile's arithmetic), its LDS layout, its
 table formats (``qs_compose_tables``) and its measurement partial layout
@@ -35,7 +38,6 @@ from . import lib as L
 from . import planner as PL
 
 PRELUDE = r"""
-#include <hip/hip_runtime.h>
 typedef long long i64;
 typedef float qs_f4 __attribute__((ext_vector_type(4)));
 struct JArgs {
@@ -50,6 +52,15 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 // constant address space lets uniform reads use the scalar (SMEM) path
 typedef const __attribute__((address_space(4))) int cint;
 __device__ __forceinline__ cint *as_const(const int *p) { return (cint *)(p); }
+// an opaque copy of a table pointer: the loads of one unitary's block cannot
+// be hoisted above it, so only that block's matrices occupy scalar registers
+// (hoisting every table read of a pass to the kernel entry runs out of SGPRs)
+__device__ __forceinline__ const int *pin(const int *p) { asm volatile("" : "+s"(p)); return p; }
+__device__ __forceinline__ float *pinf(float *p) { asm volatile("" : "+s"(p)); return p; }
+// an opaque copy of the thread index: addresses derived from it inside the
+// tile loop are recomputed where they are used instead of being hoisted out
+// of the loop and kept live in vector registers for the whole kernel
+__device__ __forceinline__ int pinv(int x) { asm volatile("" : "+v"(x)); return x; }
 // lane coefficients of a fused unitary (8 floats as int bits at u) for
 // v_mfma_f32_4x4x1f32: A_k = U[lane & 3][k] of the real 4x4 form
 __device__ __forceinline__ void coeffs(const int *up, float (&m)[4]) {
@@ -73,6 +84,62 @@ __device__ __forceinline__ void mfma_pair(float2 &x, float2 &y, const float (&m)
   x = make_float2(acc[0], acc[1]);
   y = make_float2(acc[2], acc[3]);
 }
+// coefficients of U^dagger (the transposed real 4x4 form): A_k = U[k][lane & 3]
+__device__ __forceinline__ void coeffs_dag(const int *up, float (&m)[4]) {
+  cint *u = as_const(up);
+  const float a = __int_as_float(u[0]), b = __int_as_float(u[1]), c = __int_as_float(u[2]),
+              d = __int_as_float(u[3]), e = __int_as_float(u[4]), f = __int_as_float(u[5]),
+              g = __int_as_float(u[6]), h = __int_as_float(u[7]);
+  const int row = threadIdx.x & 3;
+  const bool r1 = row & 1, r2 = row & 2;
+  m[0] = r2 ? (r1 ? -d : c) : (r1 ? -b : a);
+  m[1] = r2 ? (r1 ? c : d) : (r1 ? a : b);
+  m[2] = r2 ? (r1 ? -h : g) : (r1 ? -f : e);
+  m[3] = r2 ? (r1 ? g : h) : (r1 ? e : f);
+}
+// U^dagger on the vector ALU
+__device__ __forceinline__ void valu_pair_dag(float2 &x, float2 &y, const int *u) {
+  const float a = __int_as_float(u[0]), b = -__int_as_float(u[1]), c = __int_as_float(u[4]),
+              d = -__int_as_float(u[5]), e = __int_as_float(u[2]), f = -__int_as_float(u[3]),
+              g = __int_as_float(u[6]), h = -__int_as_float(u[7]);
+  const float2 p = x, q = y;
+  x = make_float2(a * p.x - b * p.y + c * q.x - d * q.y, a * p.y + b * p.x + c * q.y + d * q.x);
+  y = make_float2(e * p.x - f * p.y + g * q.x - h * q.y, e * p.y + f * p.x + g * q.y + h * q.x);
+}
+// Im Tr(Q M) for a gradient generator Q (8 floats as int bits) and a cross density M
+__device__ __forceinline__ float im_trace(const int *qp, const float (&m)[8]) {
+  cint *q = as_const(qp);
+  return (__int_as_float(q[0]) * m[1] + __int_as_float(q[1]) * m[0]) +
+         (__int_as_float(q[2]) * m[5] + __int_as_float(q[3]) * m[4]) +
+         (__int_as_float(q[4]) * m[3] + __int_as_float(q[5]) * m[2]) +
+         (__int_as_float(q[6]) * m[7] + __int_as_float(q[7]) * m[6]);
+}
+// sums over lane groups with DPP (no LDS traffic): quad_perm xor 1 and xor 2,
+// then row_ror 4 and 8 (every lane holds its 16-lane row sum)
+template <int CTRL>
+__device__ __forceinline__ float dpp_add(float s) {
+  return s + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(s), CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float quad_sum(float s) { return dpp_add<0x4E>(dpp_add<0xB1>(s)); }
+__device__ __forceinline__ float row_sum(float s) { return dpp_add<0x128>(dpp_add<0x124>(quad_sum(s))); }
+__device__ __forceinline__ float lane_f(float s, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(s), l));
+}
+// 64-lane sum (uniform)
+__device__ __forceinline__ float wave_sum(float s) {
+  const float r = row_sum(s);
+  return (lane_f(r, 0) + lane_f(r, 16)) + (lane_f(r, 32) + lane_f(r, 48));
+}
+// 32-lane half sums (each lane gets its half's sum)
+__device__ __forceinline__ float half_sum(float s) {
+  const float r = row_sum(s);
+  const float lo = lane_f(r, 0) + lane_f(r, 16), hi = lane_f(r, 32) + lane_f(r, 48);
+  return (threadIdx.x & 32) ? hi : lo;
+}
+struct JAdj {
+  float *psi; float *lam; const int *utab; const int *qtab; const int *cs; float *gpart; const float *seed_w;
+  i64 B; i64 NS; i64 ntiles;
+};
 // vector-ALU form (per-sample unitaries that may differ inside a 4-lane block)
 __device__ __forceinline__ void valu_pair(float2 &x, float2 &y, const int *u) {
   const float a = __int_as_float(u[0]), b = __int_as_float(u[1]), c = __int_as_float(u[2]),
@@ -85,6 +152,7 @@ __device__ __forceinline__ void valu_pair(float2 &x, float2 &y, const int *u) {
 """
 
 ARGS_FMT = "<QQQQQQqqii"   # struct JArgs
+ADJ_FMT = "<QQQQQQQqqq"    # struct JAdj
 OPTIONS = ["--offload-arch=gfx950", "-O3", "-std=c++17",
            "-mllvm", "-simplifycfg-sink-common=false",
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -150,9 +218,10 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
     e("    const i64 base = tile_base(t);")
     if not gen and not pf:
         e("    fetch(base);")
+    e("    const int ts = pinv(tid);")
     if not gen:
         for it in range(NR // 2):
-            e(f"    {{ const int sl = swz(2 * ({it * T} + tid)); const float4 val = pre[{it}]; const bool sw = sl & 1;")
+            e(f"    {{ const int sl = swz(2 * ({it * T} + ts)); const float4 val = pre[{it}]; const bool sw = sl & 1;")
             e("      *reinterpret_cast<float4 *>(&tile[sl & ~1]) = make_float4(sw ? val.z : val.x, sw ? val.w : val.y,"
               " sw ? val.x : val.z, sw ? val.y : val.w); }")
         e("    __syncthreads();")
@@ -163,8 +232,9 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
     for s, sub in enumerate(subs):
         _sub_pass(e, sub, s, len(subs), n, R, TB, multi, gen and s == 0, measure and s == len(subs) - 1)
     e("    if (A.store) {")
+    e("      const int tz = pinv(tid);")
     for it in range(NR // 2):
-        e(f"      {{ const i64 gi = base | g_tid | {goff[it]}LL; const int sl = swz(2 * ({it * T} + tid));")
+        e(f"      {{ const i64 gi = base | g_tid | {goff[it]}LL; const int sl = swz(2 * ({it * T} + tz));")
         e("        const float4 raw = *reinterpret_cast<const float4 *>(&tile[sl & ~1]); const bool sw = sl & 1;")
         e("        const float4 val = make_float4(sw ? raw.z : raw.x, sw ? raw.w : raw.y, sw ? raw.x : raw.z,"
           " sw ? raw.y : raw.w);")
@@ -183,9 +253,10 @@ def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure):
     K = R + TB
     T, NR = 1 << TB, 1 << R
     e(f"    {{  // sub-pass {s}")
+    e("      const int tq = pinv(tid);")
     e("      int gl = 0; i64 gbase = base;")
     for i in range(TB):
-        e(f"      gl |= ((tid >> {i}) & 1) << {sub.G[i]}; gbase |= (i64)((tid >> {i}) & 1) << {sub.GG[i]};")
+        e(f"      gl |= ((tq >> {i}) & 1) << {sub.G[i]}; gbase |= (i64)((tq >> {i}) & 1) << {sub.GG[i]};")
     e(f"      const i64 sample = {'gbase >> ' + str(n) if multi else 'block_sample'};")
     e("      const bool live = sample < A.B;")
     e("      const i64 csample = live ? sample : 0;")
@@ -226,13 +297,14 @@ def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure):
         for r, ui, ps in bits:
             off = f"8 * ({ui} + csample)" if ps else f"{8 * ui}"
             e("      {")
+            e("        const int *ut = pin(A.utab);")
             if ps and multi:
-                e(f"        const int *u = A.utab + {off};")
+                e(f"        const int *u = ut + {off};")
                 for j in range(NR):
                     if not (j >> r) & 1:
                         e(f"        valu_pair(v[{phys[j]}], v[{phys[j | (1 << r)]}], u);")
             else:
-                e(f"        float m[4]; coeffs(A.utab + {off}, m);")
+                e(f"        float m[4]; coeffs(ut + {off}, m);")
                 for j in range(NR):
                     if not (j >> r) & 1:
                         e(f"        mfma_pair(v[{phys[j]}], v[{phys[j | (1 << r)]}], m);")
@@ -321,14 +393,243 @@ def _measure(e, sub, phys, n, R, TB, multi):
     e("      }")
 
 
+def adjoint_source(name: str, words, n: int, R: int, TB: int, lpos, seed: bool,
+                   prefetch: bool = False, waves: int = None) -> str:
+    """HIP source of one specialised adjoint pass kernel (the adjoint tile of
+    qs_adjoint_pass with the pass program compiled in).  ``seed``: the first
+    step of the sweep, whose co-state is built from psi and w = dL/dP."""
+    K = R + TB
+    T, NR = 1 << TB, 1 << R
+    multi = n < K
+    subs = PL.decode_adjoint_pass(words, R, TB)
+    lpos = [int(x) for x in lpos]
+    npos = [] if multi else [p for p in range(n) if p not in lpos]
+    nn = len(npos)
+    seglog = (n - R) if multi else TB
+    wlog = min(seglog, 6)
+    o = []
+    e = o.append
+    wattr = f" __attribute__((amdgpu_waves_per_eu({waves})))" if waves else ""
+    e(f'extern "C" __global__ void __launch_bounds__({T}){wattr}')
+    e(f"{name}(JAdj A) {{")
+    e(f"  __shared__ __align__(16) float2 tp[{1 << K}];")
+    e(f"  __shared__ __align__(16) float2 tl[{1 << K}];")
+    e("  const int tid = threadIdx.x;")
+    e(f"  const i64 total = A.B << {n};")
+    e("  i64 g_tid = 0;")
+    for i in range(TB):
+        e(f"  g_tid |= (i64)((tid >> {i}) & 1) << {lpos[i + 1]};")
+    goff = [sum(1 << lpos[TB + 1 + i] for i in range(R - 1) if (it >> i) & 1) for it in range(NR // 2)]
+    e(f"  float4 pa[{max(NR // 2, 1)}], pb[{max(NR // 2, 1)}];")
+    e("  auto tile_base = [&](i64 t) -> i64 {")
+    if multi:
+        e(f"    return t << {K};")
+    else:
+        e(f"    i64 b = (t >> {nn}) << {n};")
+        for i, p in enumerate(npos):
+            e(f"    b |= ((t >> {i}) & 1) << {p};")
+        e("    return b;")
+    e("  };")
+    e("  auto fetch = [&](i64 base) {")
+    for it in range(NR // 2):
+        e(f"    {{ const i64 gi = base | g_tid | {goff[it]}LL;")
+        ld_a = "reinterpret_cast<const float4 *>(A.psi)[gi >> 1]"
+        ld_b = "reinterpret_cast<const float4 *>(A.lam)[gi >> 1]"
+        if multi:
+            e(f"      pa[{it}] = gi < total ? {ld_a} : make_float4(0.f, 0.f, 0.f, 0.f);")
+            if not seed:
+                e(f"      pb[{it}] = gi < total ? {ld_b} : make_float4(0.f, 0.f, 0.f, 0.f);")
+        else:
+            e(f"      pa[{it}] = {ld_a};")
+            if not seed:
+                e(f"      pb[{it}] = {ld_b};")
+        e("    }")
+    e("  };")
+    e("  i64 t = blockIdx.x;")
+    if prefetch:
+        e("  if (t < A.ntiles) fetch(tile_base(t));")
+    e("  for (; t < A.ntiles; t += gridDim.x) {")
+    e("    const i64 base = tile_base(t);")
+    if not prefetch:
+        e("    fetch(base);")
+    e(f"    const i64 block_sample = base >> {n};")
+    e("    const int ts = pinv(tid);")
+    for it in range(NR // 2):
+        e(f"    {{ const int sl = swz(2 * ({it * T} + ts)); const bool sw = sl & 1; const float4 a = pa[{it}];")
+        if seed:
+            # co-state of MeasureProbability for the pair (gi, gi + 1)
+            e(f"      const i64 gi = base | g_tid | {goff[it]}LL;")
+            e(f"      const i64 smp = {'gi >> ' + str(n) if multi else 'block_sample'};")
+            e(f"      const float *wb = A.seed_w + (gi < total ? smp : 0) * {n};")
+            e("      float w0 = 0.f;")
+            for p in range(1, n):
+                e(f"      w0 += ((gi >> {p}) & 1) ? 0.f : wb[{n - 1 - p}];")
+            e(f"      const float w1 = 2.f * w0; w0 = 2.f * (w0 + wb[{n - 1}]);")
+            e("      const float4 b = make_float4(w0 * a.x, w0 * a.y, w1 * a.z, w1 * a.w);")
+        else:
+            e(f"      const float4 b = pb[{it}];")
+        e("      *reinterpret_cast<float4 *>(&tp[sl & ~1]) = make_float4(sw ? a.z : a.x, sw ? a.w : a.y, sw ? a.x : a.z,"
+          " sw ? a.y : a.w);")
+        e("      *reinterpret_cast<float4 *>(&tl[sl & ~1]) = make_float4(sw ? b.z : b.x, sw ? b.w : b.y, sw ? b.x : b.z,"
+          " sw ? b.y : b.w); }")
+    e("    __syncthreads();")
+    if prefetch:
+        e("    { const i64 tn = t + gridDim.x; if (tn < A.ntiles) fetch(tile_base(tn)); }")
+    e(f"    const i64 uslot = t * {T >> wlog} + (tid >> {wlog});")
+    e(f"    const bool leader = (tid & {(1 << wlog) - 1}) == 0;")
+    e(f"    float2 vp[{NR}], vl[{NR}];")
+    for s, sub in enumerate(subs):
+        _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog)
+    e("    const int tz = pinv(tid);")
+    for it in range(NR // 2):
+        e(f"    {{ const i64 gi = base | g_tid | {goff[it]}LL; const int sl = swz(2 * ({it * T} + tz)); const bool sw = sl & 1;")
+        e("      const float4 a = *reinterpret_cast<const float4 *>(&tp[sl & ~1]);")
+        e("      const float4 b = *reinterpret_cast<const float4 *>(&tl[sl & ~1]);")
+        cond = "if (gi < total) " if multi else ""
+        e(f"      {cond}{{ reinterpret_cast<float4 *>(A.psi)[gi >> 1] = make_float4(sw ? a.z : a.x, sw ? a.w : a.y,"
+          " sw ? a.x : a.z, sw ? a.y : a.w);")
+        e("        reinterpret_cast<float4 *>(A.lam)[gi >> 1] = make_float4(sw ? b.z : b.x, sw ? b.w : b.y,"
+          " sw ? b.x : b.z, sw ? b.y : b.w); } }")
+    e("    __syncthreads();")
+    e("  }")
+    e("}")
+    return "\n".join(o)
+
+
+def _slot_sum(expr: str, wlog: int) -> str:
+    """Expression summing ``expr`` over the 2^wlog lanes of a reduction slot
+    (valid in the slot's first lane)."""
+    if wlog == 6:
+        return f"wave_sum({expr})"
+    if wlog == 5:
+        return f"half_sum({expr})"
+    if wlog == 4:
+        return f"row_sum({expr})"
+    if wlog == 2:
+        return f"quad_sum({expr})"
+    if wlog == 1:
+        return f"dpp_add<0xB1>({expr})"
+    if wlog == 0:
+        return expr
+    return None
+
+
+def _emit_put(e, expr: str, row: int, wlog: int, indent: str):
+    """Reduce ``expr`` over the slot's lanes and store it as gradient partial ``row``."""
+    red = _slot_sum(expr, wlog)
+    if red is not None:
+        e(f"{indent}{{ const float g = {red};")
+    else:
+        e(f"{indent}{{ float g = {expr};")
+        e(f"{indent}  for (int off = 1; off < {1 << wlog}; off <<= 1) g += __shfl_xor(g, off, 64);")
+    e(f"{indent}  if (leader) pinf(A.gpart)[(i64){row} * A.NS + uslot] = g; }}")
+
+
+def _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog):
+    NR = 1 << R
+    e(f"    {{  // adjoint sub-pass {s}")
+    e("      const int tq = pinv(tid);")
+    e("      int gl = 0; i64 gbase = base;")
+    for i in range(TB):
+        e(f"      gl |= ((tq >> {i}) & 1) << {sub.G[i]}; gbase |= (i64)((tq >> {i}) & 1) << {sub.GG[i]};")
+    e(f"      const i64 sample = {'gbase >> ' + str(n) if multi else 'block_sample'};")
+    e("      const i64 csample = sample < A.B ? sample : 0;")
+    e("      const int lb = swz(gl);")
+    sw = [_swz(1 << sub.P[r]) for r in range(R)]
+    swj = []
+    for j in range(NR):
+        a = 0
+        for r in range(R):
+            if (j >> r) & 1:
+                a ^= sw[r]
+        swj.append(a)
+    for j in range(NR):
+        e(f"      vp[{j}] = tp[lb ^ {swj[j]}]; vl[{j}] = tl[lb ^ {swj[j]}];")
+
+    phys = list(range(NR))
+    if sub.phases:
+        terms = " + ".join(f"(vl[{j}].x * vp[{j}].y - vl[{j}].y * vp[{j}].x)" for j in range(NR))
+        e(f"      {{ const float S = {terms};")
+        for pos, ci, fl, row in sub.phases:
+            if row >= 0:
+                _emit_put(e, f"(((gbase >> {pos}) & 1) ? -S : S)", row, wlog, "        ")
+        e("        float pr = 1.f, pi = 0.f;")
+        for pos, ci, fl, row in sub.phases:
+            if fl & PL.SEG_NOUNAPPLY:
+                continue
+            idx = f"({ci} + csample)" if fl & PL.SEG_PER_SAMPLE else f"{ci}"
+            e(f"        {{ const float c = __int_as_float(as_const(A.cs)[2 * {idx}]), "
+              f"sn0 = __int_as_float(as_const(A.cs)[2 * {idx} + 1]);")
+            # inverse phase: the sign flips
+            e(f"          const float sn = ((gbase >> {pos}) & 1) ? sn0 : -sn0;")
+            e("          const float nr = pr * c + pi * sn, ni = pi * c - pr * sn; pr = nr; pi = ni; }")
+        e("        const float2 z = make_float2(pr, pi);")
+        for j in range(NR):
+            e(f"        vp[{j}] = cmul(z, vp[{j}]); vl[{j}] = cmul(z, vl[{j}]);")
+        e("      }")
+    for cnots, bits in sub.segments:
+        for uniform, tgt, ctl in cnots:
+            if not uniform:
+                for j in range(NR):
+                    if (j >> ctl) & 1 and not (j >> tgt) & 1:
+                        k = j | (1 << tgt)
+                        phys[j], phys[k] = phys[k], phys[j]
+            else:
+                e(f"      {{ const bool cnd = (gbase >> {ctl}) & 1;")
+                for j in range(NR):
+                    if not (j >> tgt) & 1:
+                        a, b = phys[j], phys[j | (1 << tgt)]
+                        e(f"        {{ const float2 x = vp[{a}], y = vp[{b}]; vp[{a}] = cnd ? y : x; vp[{b}] = cnd ? x : y; }}")
+                        e(f"        {{ const float2 x = vl[{a}], y = vl[{b}]; vl[{a}] = cnd ? y : x; vl[{b}] = cnd ? x : y; }}")
+                e("      }")
+        for r, u, ps, probe, q, rows in bits:
+            e("      {")
+            e("        const int *ut = pin(A.utab), *qt = pin(A.qtab);")
+            e("        float m[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};")
+            for j in range(NR):
+                if not (j >> r) & 1:
+                    a, b = phys[j], phys[j | (1 << r)]
+                    e(f"        {{ const float2 p0 = vp[{a}], p1 = vp[{b}], l0 = vl[{a}], l1 = vl[{b}];")
+                    e("          m[0] += p0.x * l0.x + p0.y * l0.y; m[1] += p0.y * l0.x - p0.x * l0.y;")
+                    e("          m[2] += p0.x * l1.x + p0.y * l1.y; m[3] += p0.y * l1.x - p0.x * l1.y;")
+                    e("          m[4] += p1.x * l0.x + p1.y * l0.y; m[5] += p1.y * l0.x - p1.x * l0.y;")
+                    e("          m[6] += p1.x * l1.x + p1.y * l1.y; m[7] += p1.y * l1.x - p1.x * l1.y; }")
+            span = "A.B" if ps else "1"
+            var = "csample" if ps else "0"
+            for k, row in enumerate(rows):
+                if row < 0:
+                    continue
+                _emit_put(e, f"im_trace(qt + 8 * ({q} + {k} * {span} + {var}), m)", row, wlog, "        ")
+            if not probe:
+                off = f"8 * ({u} + csample)" if ps else f"{8 * u}"
+                if ps and multi:
+                    e(f"        const int *uu = ut + {off};")
+                    for j in range(NR):
+                        if not (j >> r) & 1:
+                            a, b = phys[j], phys[j | (1 << r)]
+                            e(f"        valu_pair_dag(vp[{a}], vp[{b}], uu); valu_pair_dag(vl[{a}], vl[{b}], uu);")
+                else:
+                    e(f"        float cf[4]; coeffs_dag(ut + {off}, cf);")
+                    for j in range(NR):
+                        if not (j >> r) & 1:
+                            a, b = phys[j], phys[j | (1 << r)]
+                            e(f"        mfma_pair(vp[{a}], vp[{b}], cf); mfma_pair(vl[{a}], vl[{b}], cf);")
+            e("      }")
+    for j in range(NR):
+        e(f"      tp[lb ^ {swj[j]}] = vp[{phys[j]}]; tl[lb ^ {swj[j]}] = vl[{phys[j]}];")
+    e("      __syncthreads();")
+    e("    }")
+
+
 # ---------------------------------------------------------------------------
 # compile / cache / launch
 # ---------------------------------------------------------------------------
 
 _lock = threading.Lock()
-_modules: dict = {}          # source hash -> (module handle, {name: function})
+_kernels: dict = {}          # program source hash -> (module handle, {kernel name: function handle})
 CACHE_DIR = os.environ.get("QANDLE_AMD_JIT_CACHE",
                            os.path.join(os.path.expanduser("~"), ".cache", "qandle_amd", "jit"))
+KERNEL = "qs_pass"
 
 
 def _compile(src: str, name: str) -> bytes:
@@ -354,41 +655,59 @@ def _compile(src: str, name: str) -> bytes:
             lib.qs_jit_release_code(h)
 
 
-def module_for(src: str, names):
-    """Loaded module holding the kernels ``names`` of ``src`` (compiled or from the disk cache)."""
-    key = hashlib.sha256((PRELUDE + src + " ".join(OPTIONS)).encode()).hexdigest()[:32]
+def _key(src: str) -> str:
+    return hashlib.sha256((PRELUDE + src + " ".join(OPTIONS)).encode()).hexdigest()[:32]
+
+
+def _code_for(src: str) -> bytes:
+    """Code object of one kernel source: from the disk cache, else compiled and stored."""
+    key = _key(src)
+    path = os.path.join(CACHE_DIR, key + ".hsaco")
+    try:
+        with open(path, "rb") as f:
+            code = f.read()
+        if code:
+            return code
+    except OSError:
+        pass
+    code = _compile(PRELUDE + src, "qs_jit_" + key)
+    try:
+        os.makedirs(CACHE_DIR, exist_ok=True)
+        tmp = path + f".{os.getpid()}.{threading.get_ident()}.tmp"
+        with open(tmp, "wb") as f:
+            f.write(code)
+        os.replace(tmp, path)
+    except OSError:
+        pass
+    return code
+
+
+def kernels_for(sources, names):
+    """Function handles of the kernels ``names`` defined by ``sources``.
+
+    All sources form one program (hipRTC serialises compilations, and one
+    program pays the fixed compiler start-up once); its code object is cached
+    on disk by source hash and loaded once per process."""
+    src = "\n\n".join(sources)
+    key = _key(src)
     with _lock:
-        hit = _modules.get(key)
-        if hit is not None:
-            return hit
-        path = os.path.join(CACHE_DIR, key + ".hsaco")
-        code = None
-        try:
-            with open(path, "rb") as f:
-                code = f.read()
-        except OSError:
-            pass
-        if not code:
-            code = _compile(PRELUDE + src, "qs_jit_" + key)
-            try:
-                os.makedirs(CACHE_DIR, exist_ok=True)
-                tmp = path + f".{os.getpid()}.tmp"
-                with open(tmp, "wb") as f:
-                    f.write(code)
-                os.replace(tmp, path)
-            except OSError:
-                pass
+        hit = _kernels.get(key)
+    if hit is None:
+        code = _code_for(src)
         lib = L.load()
-        mod = ctypes.c_void_p()
-        L.check(lib.qs_jit_load(code, len(code), ctypes.byref(mod)), "qs_jit_load")
-        funcs = {}
-        for nm in names:
-            fn = ctypes.c_void_p()
-            L.check(lib.qs_jit_function(mod, nm.encode(), ctypes.byref(fn)), f"qs_jit_function({nm})")
-            funcs[nm] = fn
-        hit = (mod, funcs)
-        _modules[key] = hit
-        return hit
+        with _lock:
+            hit = _kernels.get(key)
+            if hit is None:
+                mod = ctypes.c_void_p()
+                L.check(lib.qs_jit_load(code, len(code), ctypes.byref(mod)), "qs_jit_load")
+                fns = {}
+                for nm in names:
+                    fn = ctypes.c_void_p()
+                    L.check(lib.qs_jit_function(mod, nm.encode(), ctypes.byref(fn)), f"qs_jit_function({nm})")
+                    fns[nm] = fn
+                hit = (mod, fns)
+                _kernels[key] = hit
+    return [hit[1][nm] for nm in names]
 
 
 class ProgramKernels:
@@ -398,18 +717,15 @@ class ProgramKernels:
                  prefetch: bool = True, waves: int = None):
         self.n, self.R, self.TB = n, R, TB
         self.block = 1 << TB
-        self.names = []
-        parts = []
         npass = len(passes)
+        self.sources, self.names = [], []
         for i, (w, _nsub, lpos) in enumerate(passes):
             words = w.cpu().numpy() if hasattr(w, "cpu") else w
-            nm = f"qs_pass_{i}"
-            parts.append(pass_source(nm, words, n, R, TB, lpos, gen_first and i == 0, measure and i == npass - 1,
-                                     prefetch, waves))
+            nm = f"{KERNEL}_{i}"
+            self.sources.append(pass_source(nm, words, n, R, TB, lpos, gen_first and i == 0,
+                                            measure and i == npass - 1, prefetch, waves))
             self.names.append(nm)
-        self.source = "\n\n".join(parts)
-        self.mod, funcs = module_for(self.source, self.names)
-        self.funcs = [funcs[nm] for nm in self.names]
+        self.funcs = kernels_for(self.sources, self.names)
         self.grid_cap = [None] * len(self.funcs)
 
     def launch(self, i: int, state, src, utab, cs, efac, mpart, B: int, ntiles: int, store: int, stream):
@@ -420,5 +736,31 @@ class ProgramKernels:
         grid = max(1, min(ntiles, self.grid_cap[i]))
         args = struct.pack(ARGS_FMT, state or 0, src or 0, utab or 0, cs or 0, efac or 0, mpart or 0,
                            B, ntiles, store, 0)
+        buf = ctypes.create_string_buffer(args, len(args))
+        L.check(lib.qs_jit_launch(self.funcs[i], grid, self.block, buf, len(args), stream), "qs_jit_launch")
+
+
+class AdjointKernels:
+    """The specialised kernels of every step of one encoded adjoint sweep."""
+
+    def __init__(self, passes, n: int, R: int, TB: int, seed_first: bool, prefetch: bool = False,
+                 waves: int = None):
+        self.block = 1 << TB
+        self.sources, self.names = [], []
+        for i, (w, _nsub, lpos) in enumerate(passes):
+            words = w.cpu().numpy() if hasattr(w, "cpu") else w
+            nm = f"qs_adj_{i}"
+            self.sources.append(adjoint_source(nm, words, n, R, TB, lpos, seed_first and i == 0, prefetch, waves))
+            self.names.append(nm)
+        self.funcs = kernels_for(self.sources, self.names)
+        self.grid_cap = [None] * len(self.funcs)
+
+    def launch(self, i: int, psi, lam, utab, qtab, cs, gpart, seed_w, B: int, NS: int, ntiles: int, stream):
+        lib = L.load()
+        if self.grid_cap[i] is None:
+            g = lib.qs_jit_resident_blocks(self.funcs[i], self.block)
+            self.grid_cap[i] = int(g) if g and g > 0 else 2048
+        grid = max(1, min(ntiles, self.grid_cap[i]))
+        args = struct.pack(ADJ_FMT, psi, lam, utab or 0, qtab or 0, cs or 0, gpart, seed_w or 0, B, NS, ntiles)
         buf = ctypes.create_string_buffer(args, len(args))
         L.check(lib.qs_jit_launch(self.funcs[i], grid, self.block, buf, len(args), stream), "qs_jit_launch")

@@ -425,3 +425,42 @@ def decode_pass(words, R: int, TB: int):
     if pos != len(w):
         raise ValueError("malformed pass words")
     return out
+
+
+def decode_adjoint_pass(words, R: int, TB: int):
+    """Adjoint (reverse-encoded) words of one pass -> sub-pass structures.
+
+    Segments are (cnots, bits) with cnots = [(uniform, target, control)] and
+    bits = [(r, utab index, per_sample, probe, qtab index, [rows])]; phases
+    are (position, cs index, flags, row)."""
+    w = [int(x) for x in words]
+    pos, out = 0, []
+    while pos < len(w):
+        hdr = w[pos: pos + HDR]
+        nseg, nph = hdr[0], hdr[6]
+        sub = DecodedSub([hdr[HDR_P + r] for r in range(R)], [hdr[HDR_G + i] for i in range(TB)],
+                         [hdr[HDR_GP + r] for r in range(R)], [hdr[HDR_GG + i] for i in range(TB)], [], [])
+        p = pos + HDR
+        for _ in range(nph):
+            sub.phases.append((w[p], w[p + 1], w[p + 2], w[p + 3]))
+            p += 4
+        for _ in range(nseg):
+            head = w[p]
+            umask, ncnot = head & 0xFF, (head >> 8) & 0xFF
+            psmask, probemask = (head >> 16) & 0xFF, (head >> 24) & 0x7F
+            p += 1
+            cn = [(bool(cw & CNOT_UNIFORM), (cw >> 8) & 0xFF, cw >> 16) for cw in w[p: p + ncnot]]
+            p += ncnot
+            bits = []
+            for r in range(R):
+                if (umask >> r) & 1:
+                    u, nc, q = w[p], w[p + 1], w[p + 2]
+                    rows = w[p + 3: p + 3 + nc]
+                    bits.append((r, u, bool((psmask >> r) & 1), bool((probemask >> r) & 1), q, rows))
+                    p += 3 + nc
+            sub.segments.append((cn, bits))
+        out.append(sub)
+        pos = p
+    if pos != len(w):
+        raise ValueError("malformed adjoint pass words")
+    return out

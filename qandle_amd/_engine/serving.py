@@ -17,6 +17,8 @@ server puts in front of ``Circuit.forward``:
 * Results are a pure function of (circuit, inputs): the kernel shape and
   fusion plan do not depend on load, and every reduction has a fixed order, so
   a request computes the same bits alone and in a mix.
+* Requests run on the library's interpreting pass kernels: a new circuit is
+  served at once instead of waiting for per-program kernels to compile.
 * Latency = completion time - arrival time (host clock); p50/p99 reported with
   throughput.
 """
@@ -166,7 +168,8 @@ class ServingEngine:
             R, TB = fused.choose_shape(prog.n)
             cs = fused.build_coeffs(prog, inp)
             efac = fused.product_factors(prog, inp)
-            _, probs = fused.execute(prog, inp, cs, efac, R, TB, want_state=False)
+            # interpreting kernels: a request never waits for a program to compile
+            _, probs = fused.execute(prog, inp, cs, efac, R, TB, want_state=False, specialise=False)
             ch.out = probs
             ch.keep = (cs, efac)
             ev = torch.cuda.Event()

@@ -13,5 +13,7 @@ CHECKPOINT_LAYERS = 5   # adjoint backward restores psi from a checkpoint every 
 CHECKPOINT_GATES = None  # explicit checkpoint spacing in gates (overrides CHECKPOINT_LAYERS)
 AUTOTUNE = False         # time the kernel shape / fusion budget candidates on first use per (circuit, batch)
 AUTOTUNE_CACHE = None    # JSON file that keeps tuned choices across processes
-SPECIALISE = False       # forward passes run as kernels compiled at run time for the pass program (hipRTC)
-SPECIALISE_OPTIONS = {}  # generator options of those kernels: prefetch (bool), waves (int or None)
+SPECIALISE = True        # passes run as kernels compiled at run time for the pass program (hipRTC);
+                         # False runs the interpreting pass kernels of the library
+SPECIALISE_OPTIONS = {"prefetch": True}  # forward kernel options: prefetch (bool), waves (int or None)
+SPECIALISE_ADJOINT_OPTIONS = {}  # adjoint sweep kernel options: prefetch (bool), waves (int or None)

@@ -29,6 +29,9 @@ FLAGS = [
     # dispatch switches into a common block: that turns the register-array
     # index into a phi and forces the whole array into scratch memory
     "-mllvm", "-simplifycfg-sink-common=false",
+    # matrix-core (MFMA) results stay in vector registers: no copies between
+    # accumulator and vector registers around every fused unitary
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wno-unused-command-line-argument",
 ]
 
