@@ -26,8 +26,9 @@ def _needs_grad(inp) -> bool:
 
 
 def forward(prog, inp):
-    R, TB, cm = fused.choose_config(prog, inp)
-    if not _needs_grad(inp):
+    grad = _needs_grad(inp)
+    R, TB, cm = fused.choose_config(prog, inp, grad)
+    if not grad:
         cs = fused.build_coeffs(prog, inp)
         efac = fused.product_factors(prog, inp) if (prog.init[0] == "embed" or inp.state is None) else None
         psi, probs = fused.execute(prog, inp, cs, efac, R, TB, cm=cm, want_state=not prog.measure)

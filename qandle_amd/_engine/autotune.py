@@ -25,9 +25,18 @@ _memo = {}
 
 
 def candidates(n: int):
-    """(R, TB, budget, min_low) candidates compiled into the library for width n."""
+    """(R, TB, budget, min_low) candidates for width n.
+
+    With specialised kernels every candidate is a compilation, so the list is
+    short and spans the fusion budgets those kernels favour; the interpreting
+    kernels are tuned over lighter budgets and the always-local low bits."""
     out = []
     shapes = [(5, 6), (4, 8), (5, 8), (4, 7)] if n >= 5 else [(n, 8)]
+    if getattr(config, "SPECIALISE", False):
+        for R, TB in shapes[:3]:
+            if R <= n:
+                out += [(R, TB, 256.0, 4), (R, TB, 1024.0, 4)]
+        return out
     for R, TB in shapes:
         if R > n:
             continue

@@ -152,11 +152,16 @@ def get_plan(prog: Program, R: int, TB: int, cm: PL.CostModel = None) -> PL.Plan
     return pl
 
 
-DEFAULT_COST = PL.CostModel(budget=256.0, min_low=4)
+# fusion budgets: the specialised kernels make per-gate work cheap, so passes
+# are packed to the planner's structural limit; the interpreting kernels pay
+# a dispatch per op and are kept lighter
+DEFAULT_COST = PL.CostModel(budget=1024.0, min_low=4)
+INTERPRETED_COST = PL.CostModel(budget=256.0, min_low=4)
 
 
-def choose_config(prog: Program, inp):
-    """(R, TB, cost model) for a call: the autotuned choice when enabled, else defaults."""
+def choose_config(prog: Program, inp, grad: bool = False):
+    """(R, TB, cost model) for a call: the autotuned choice when enabled, else
+    the defaults for a forward-only call or, with ``grad``, for forward + adjoint."""
     from .. import config
 
     if getattr(config, "AUTOTUNE", False):
@@ -167,21 +172,31 @@ def choose_config(prog: Program, inp):
             with torch.no_grad():
                 hit, _ = autotune.tune(prog, inp)
         return hit
-    R, TB = choose_shape(prog.n)
+    R, TB = choose_shape(prog.n, grad)
     return R, TB, DEFAULT_COST
 
 
-def choose_shape(n: int):
-    """(R, TB) for a circuit width: from 12 qubits, tiles of 2^12 amplitudes
-    over 256 threads with 16 amplitudes per thread (the widest tile whose
-    passes and adjoint sweeps keep two or more waves per SIMD); from 5 qubits,
-    2^11-amplitude tiles of one wave with 32 amplitudes per thread; below
-    that, every bit in registers of a 256-thread tile holding several samples."""
+def choose_shape(n: int, grad: bool = False):
+    """(R, TB) of the specialised kernels for a circuit width.
+
+    Forward only: from 13 qubits, tiles of 2^13 amplitudes over 256 threads
+    with 32 amplitudes per thread (the fewest passes).  With gradients: from
+    12 qubits, 2^12-amplitude tiles with 16 amplitudes per thread (the adjoint
+    holds psi and lambda, and this shape keeps two waves per SIMD).  From 5
+    qubits, 2^11-amplitude tiles of one wave; below that, every bit in
+    registers of a 256-thread tile holding several samples."""
+    if n >= 13 and not grad:
+        return 5, 8
     if n >= 12:
         return 4, 8
     if n >= 5:
         return 5, 6
     return n, DEFAULT_TB
+
+
+def interpreted_shape(n: int):
+    """(R, TB) of the interpreting kernels (serving): one-wave 2^11-amplitude tiles."""
+    return (5, 6) if n >= 5 else (n, DEFAULT_TB)
 
 
 # ----------------------------------------------------------------------------

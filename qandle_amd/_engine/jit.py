@@ -163,7 +163,7 @@ def _swz(l: int) -> int:
 
 
 def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, measure: bool,
-                prefetch: bool = True, waves: int = None) -> str:
+                prefetch: bool = False, waves: int = None) -> str:
     """HIP source of one specialised forward pass kernel (extern "C" ``name``).
 
     ``prefetch`` keeps the next tile's HBM loads in flight in registers while
@@ -714,7 +714,7 @@ class ProgramKernels:
     """The specialised kernels of every pass of one encoded forward program."""
 
     def __init__(self, passes, n: int, R: int, TB: int, gen_first: bool, measure: bool,
-                 prefetch: bool = True, waves: int = None):
+                 prefetch: bool = False, waves: int = None):
         self.n, self.R, self.TB = n, R, TB
         self.block = 1 << TB
         npass = len(passes)

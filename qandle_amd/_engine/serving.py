@@ -102,8 +102,8 @@ class ServingEngine:
         x = req.inputs[prog.init[3]] if prog.init[0] == "embed" else None
         req.n = prog.n
         req.B = 1 if x is None or x.dim() == 1 else x.shape[0]
-        R, TB = fused.choose_shape(prog.n)
-        fused.get_plan(prog, R, TB)
+        R, TB = fused.interpreted_shape(prog.n)
+        fused.get_plan(prog, R, TB, fused.INTERPRETED_COST)
         per = state_footprint(req.n, 1, R, TB)
         req.footprint = state_footprint(req.n, req.B, R, TB)
         if per > self.cap:
@@ -120,7 +120,7 @@ class ServingEngine:
             return
         req.t_queued = self.now()
         for lo, hi in req.chunks:
-            R, TB = fused.choose_shape(req.n)
+            R, TB = fused.interpreted_shape(req.n)
             self.queue.append(_Chunk(req, lo, hi, state_footprint(req.n, hi - lo, R, TB)))
         self._admit()
 
@@ -165,11 +165,12 @@ class ServingEngine:
                 x = kw[prog.init[3]]
                 kw[prog.init[3]] = x[ch.lo:ch.hi] if x.dim() == 2 else x
             inp = fused.gather_inputs(prog, req.circuit.circuit, None, kw)
-            R, TB = fused.choose_shape(prog.n)
+            R, TB = fused.interpreted_shape(prog.n)
             cs = fused.build_coeffs(prog, inp)
             efac = fused.product_factors(prog, inp)
             # interpreting kernels: a request never waits for a program to compile
-            _, probs = fused.execute(prog, inp, cs, efac, R, TB, want_state=False, specialise=False)
+            _, probs = fused.execute(prog, inp, cs, efac, R, TB, cm=fused.INTERPRETED_COST, want_state=False,
+                                     specialise=False)
             ch.out = probs
             ch.keep = (cs, efac)
             ev = torch.cuda.Event()

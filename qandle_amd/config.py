@@ -15,5 +15,5 @@ AUTOTUNE = False         # time the kernel shape / fusion budget candidates on f
 AUTOTUNE_CACHE = None    # JSON file that keeps tuned choices across processes
 SPECIALISE = True        # passes run as kernels compiled at run time for the pass program (hipRTC);
                          # False runs the interpreting pass kernels of the library
-SPECIALISE_OPTIONS = {"prefetch": True}  # forward kernel options: prefetch (bool), waves (int or None)
+SPECIALISE_OPTIONS = {}  # forward kernel options: prefetch (bool), waves (int or None)
 SPECIALISE_ADJOINT_OPTIONS = {}  # adjoint sweep kernel options: prefetch (bool), waves (int or None)
