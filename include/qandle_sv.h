@@ -152,6 +152,18 @@ int64_t qs_fused_measure_work_bytes(int64_t B, int n, int R, int TB);
 int qs_fused_measure_finish(float *out, const float *mpart, int64_t B, int n,
                             int R, int TB, const int32_t *lpos, void *stream);
 
+/* MeasureProbability from readout partials: every tile (or, for tiles that
+ * hold several samples, every sample of a tile) wrote n + 1 floats, entry p =
+ * its share of P(measured bit at position p = 0) and entry n its total; out
+ * (B, n) float32 sums the shares of each sample's tiles.  This is the form of
+ * the run-time specialised kernels, whose measured bit may be a parity of
+ * state bits (CNOTs that end a circuit are folded into its readout).
+ * Replaces src/qandle/measurements.py:120-123 together with the trailing
+ * CNOT layers of src/qandle/operators.py:557-558. */
+int64_t qs_readout_work_bytes(int64_t B, int n, int R, int TB);
+int qs_readout_finish(float *out, const float *mpart, int64_t B, int n, int R,
+                      int TB, void *stream);
+
 /* ---------------------------------------------------------------------------
  * Adjoint backward.  Replaces torch autograd through every dense gate of the
  * reference (implicit; the tape of src/qandle/operators.py:272-298 and the

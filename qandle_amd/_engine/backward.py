@@ -7,6 +7,8 @@ Backward (reverse sweep over the same passes, ``qs_adjoint_pass``):
 
 * co-state seed: lambda = 2 psi * sum_q w_q [bit n-1-q = 0] for a
   MeasureProbability output (w = dL/dP), or dL/dpsi for a state output;
+  when the compiled kernels fold trailing CNOTs into the readout, psi is the
+  state before them and the measured bit is the parity of its readout mask;
 * per pass, in reverse: for every gate, 1/2 Im<lambda|P|psi> at its output,
   then psi <- U^dag psi, lambda <- U^dag lambda;
 * where a checkpoint exists, psi is reset to it, which bounds the float32
@@ -55,7 +57,9 @@ class FusedCircuitFn(torch.autograd.Function):
                            None if state is None else state.detach())
         cs = fused.build_coeffs(prog, inp)
         efac = fused.product_factors(prog, inp) if (prog.init[0] == "embed" or state is None) else None
-        pl = fused.get_plan(prog, R, TB, cm)
+        jit_on = fused.use_jit(prog.n, R, TB)
+        fold = fused.folds(prog, jit_on)
+        pl = fused.get_plan(prog, R, TB, cm, fold)
         every = config.CHECKPOINT_GATES if getattr(config, "CHECKPOINT_GATES", None) else default_checkpoint_gates(prog.n)
         marks = checkpoint_after(prog, pl, every)
         ckpts = {}
@@ -64,8 +68,10 @@ class FusedCircuitFn(torch.autograd.Function):
             if i in marks:
                 ckpts[i] = psi.clone()
 
-        psi, probs = fused.execute(prog, inp, cs, efac, R, TB, cm=cm, want_state=True, checkpoints=keep)
+        psi, probs = fused.execute(prog, inp, cs, efac, R, TB, cm=cm, want_state=True, checkpoints=keep,
+                                   specialise=jit_on)
         ctx.prog, ctx.R, ctx.TB, ctx.cm, ctx.B, ctx.names = prog, R, TB, cm, B, names
+        ctx.jit, ctx.fold = jit_on, fold
         ctx.ck_index = sorted(ckpts)
         ctx.named_shapes = [v.shape for v in named_vals]
         ctx.embed_shape = None if embed_x is None else embed_x.shape
@@ -92,7 +98,7 @@ class FusedCircuitFn(torch.autograd.Function):
             seed = gout.to(torch.float32).contiguous()
         else:
             lam.copy_(gout.to(torch.complex64).resolve_conj())
-        pl = fused.get_plan(prog, R, TB, ctx.cm)
+        pl = fused.get_plan(prog, R, TB, ctx.cm, ctx.fold)
         rows = fused.rotation_rows(prog)
         nrows = len(rows)
         embed = prog.init[0] == "embed" and ctx.needs_input_grad[8]
@@ -104,8 +110,8 @@ class FusedCircuitFn(torch.autograd.Function):
         gpart = torch.empty(max(wb // 4, 1), dtype=torch.float32, device=psi.device)
         npass = len(pl.passes)
         kern = None
-        if fused.use_jit(n, R, TB):
-            kern = specialised_adjoint(prog, pl, ws, B, embed, seed is not None)
+        if ctx.jit:
+            kern = specialised_adjoint(prog, pl, ws, B, embed, seed is not None, ctx.fold)
             ntiles, NS = adjoint_geometry(B, n, R, TB)
         for k, (w, nsub, lpos) in enumerate(ws.passes):
             seed_ptr = None if (k > 0 or seed is None) else L.ptr(seed)
@@ -176,15 +182,18 @@ def adjoint_geometry(B: int, n: int, R: int, TB: int):
     return ntiles, ntiles * (1 << (TB - wlog))
 
 
-def specialised_adjoint(prog, pl, ws, B, embed, seeded):
-    """Run-time compiled kernels of this plan's adjoint sweep (cached on the program)."""
+def specialised_adjoint(prog, pl, ws, B, embed, seeded, fold=False):
+    """Run-time compiled kernels of this plan's adjoint sweep (cached on the
+    program); with ``fold`` the seed reads the readout masks of the folded
+    trailing CNOTs (fused.readout_fold)."""
     from . import jit
 
     opts = getattr(config, "SPECIALISE_ADJOINT_OPTIONS", {}) or {}
-    key = ("jit_adj", id(pl), B, embed, seeded, tuple(sorted(opts.items())))
+    key = ("jit_adj", id(pl), B, embed, seeded, fold, tuple(sorted(opts.items())))
     hit = prog.words_cache.get(key)
     if hit is None:
-        hit = jit.AdjointKernels(ws.passes, prog.n, pl.R, pl.TB, seeded, **opts)
+        masks = fused.readout_fold(prog)[1] if fold else None
+        hit = jit.AdjointKernels(ws.passes, prog.n, pl.R, pl.TB, seeded, readout=masks, **opts)
         prog.words_cache[key] = hit
     return hit
 

@@ -142,14 +142,57 @@ def program_from_spec(n: int, gates, embed=None, measure: bool = True) -> Progra
     return Program(n, pg, slots, init, measure)
 
 
-def get_plan(prog: Program, R: int, TB: int, cm: PL.CostModel = None) -> PL.Plan:
+def readout_fold(prog: Program):
+    """(number of gates kept, readout masks) when CNOTs end a measured program,
+    else None.
+
+    A CNOT maps basis state x to x with its target bit XOR-ed by its control
+    bit, so a network of CNOTs is an invertible linear map over GF(2) and the
+    probabilities after it are those before it, permuted.  MeasureProbability
+    of the bit at position p after the network is therefore P(parity(a_p & x)
+    = 0) of the state x before it, with a_p (a mask over bit positions) read
+    off the network: starting from a_p = 1 << p, each CNOT from the last to
+    the first adds its control to every mask that holds its target.  The
+    trailing CNOTs of a circuit (the entangling layer that ends
+    src/qandle/operators.py:557-558 ladders) then cost no pass at all."""
+    if not prog.measure:
+        return None
+    hit = prog.words_cache.get("readout_fold", False)
+    if hit is not False:
+        return hit
+    m = len(prog.gates)
+    while m > 0 and prog.gates[m - 1].kind == PL.CNOT:
+        m -= 1
+    hit = None
+    if 0 < m < len(prog.gates):
+        masks = [1 << p for p in range(prog.n)]
+        for g in reversed(prog.gates[m:]):
+            for p in range(prog.n):
+                if (masks[p] >> g.pos) & 1:
+                    masks[p] ^= 1 << g.ctrl
+        hit = (m, masks)
+    prog.words_cache["readout_fold"] = hit
+    return hit
+
+
+def get_plan(prog: Program, R: int, TB: int, cm: PL.CostModel = None, fold: bool = False) -> PL.Plan:
+    """Plan of the program's gates, or with ``fold`` of the gates that
+    readout_fold keeps (the trailing CNOTs then live in the readout masks)."""
     cm = cm or DEFAULT_COST
-    key = (R, TB, None if cm is None else dataclasses.astuple(cm))
+    rf = readout_fold(prog) if fold else None
+    key = (R, TB, None if cm is None else dataclasses.astuple(cm), rf is not None)
     pl = prog.plans.get(key)
     if pl is None:
-        pl = PL.plan(prog.n, prog.gates, R, TB, cm)
+        gates = prog.gates if rf is None else prog.gates[:rf[0]]
+        pl = PL.plan(prog.n, gates, R, TB, cm)
         prog.plans[key] = pl
     return pl
+
+
+def folds(prog: Program, specialise: bool) -> bool:
+    """Whether a run folds the program's trailing CNOTs into its readout (the
+    run-time specialised kernels only)."""
+    return bool(specialise) and readout_fold(prog) is not None
 
 
 # fusion budgets: the specialised kernels make per-gate work cheap, so passes
@@ -514,16 +557,18 @@ def use_jit(n: int, R: int, TB: int) -> bool:
     return bool(getattr(config, "SPECIALISE", False))
 
 
-def specialised_kernels(prog: Program, pl: PL.Plan, ws: WordSet, B: int, gen_first: bool):
-    """Run-time compiled kernels of this plan's passes (cached on the program)."""
+def specialised_kernels(prog: Program, pl: PL.Plan, ws: WordSet, B: int, gen_first: bool, fold: bool = False):
+    """Run-time compiled kernels of this plan's passes (cached on the program);
+    with ``fold`` the last pass writes readout partials (readout_fold)."""
     from .. import config
     from . import jit
 
     opts = getattr(config, "SPECIALISE_OPTIONS", {}) or {}
-    key = (id(pl), B, gen_first, prog.measure, tuple(sorted(opts.items())))
+    key = (id(pl), B, gen_first, prog.measure, fold, tuple(sorted(opts.items())))
     hit = prog.words_cache.get(("jit",) + key)
     if hit is None:
-        hit = jit.ProgramKernels(ws.passes, prog.n, pl.R, pl.TB, gen_first, prog.measure, **opts)
+        masks = readout_fold(prog)[1] if fold else None
+        hit = jit.ProgramKernels(ws.passes, prog.n, pl.R, pl.TB, gen_first, prog.measure, readout=masks, **opts)
         prog.words_cache[("jit",) + key] = hit
     return hit
 
@@ -536,9 +581,13 @@ def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
     ``events``: optional list; a (start, end) pair of HIP events recorded on the
     launch stream around every pass kernel is appended to it.  ``specialise``:
     run compiled-per-program kernels (None: config.SPECIALISE) or the library's
-    interpreting kernels (False; no compilation on first use of a program)."""
+    interpreting kernels (False; no compilation on first use of a program).
+    Compiled kernels fold trailing CNOTs into the readout of a measured
+    program (readout_fold): the state they leave is the one before those CNOTs."""
     n, B = prog.n, inp.B
-    pl = get_plan(prog, R, TB, cm)
+    jit_on = use_jit(n, R, TB) if specialise is None else specialise
+    fold = folds(prog, jit_on)
+    pl = get_plan(prog, R, TB, cm, fold)
     ws = pass_words(prog, pl, B, inp.device)
     words = ws.passes
     lib = L.load()
@@ -555,11 +604,10 @@ def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
         src = s.contiguous()
     mpart = probs = None
     if prog.measure:
-        wb = lib.qs_fused_measure_work_bytes(B, n, R, TB)
+        wb = (lib.qs_readout_work_bytes if fold else lib.qs_fused_measure_work_bytes)(B, n, R, TB)
         mpart = torch.empty(max(wb // 4, 1), dtype=torch.float32, device=inp.device)
     npass = len(words)
-    jit_on = use_jit(n, R, TB) if specialise is None else specialise
-    kern = specialised_kernels(prog, pl, ws, B, src is None) if jit_on else None
+    kern = specialised_kernels(prog, pl, ws, B, src is None, fold) if jit_on else None
     ntiles = tiles_of(B, n, R + TB)
     for i, (w, nsub, lpos) in enumerate(words):
         last = i == npass - 1
@@ -587,7 +635,10 @@ def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
         src = psi
         if checkpoints is not None:
             checkpoints(i, psi)
-    if prog.measure:
+    if prog.measure and fold:
+        probs = torch.empty(B, n, dtype=torch.float32, device=inp.device)
+        L.check(lib.qs_readout_finish(L.ptr(probs), L.ptr(mpart), B, n, R, TB, st), "qs_readout_finish")
+    elif prog.measure:
         probs = torch.empty(B, n, dtype=torch.float32, device=inp.device)
         lpos = words[-1][2]
         L.check(

@@ -163,12 +163,16 @@ def _swz(l: int) -> int:
 
 
 def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, measure: bool,
-                prefetch: bool = False, waves: int = None) -> str:
+                prefetch: bool = False, waves: int = None, readout=None) -> str:
     """HIP source of one specialised forward pass kernel (extern "C" ``name``).
 
     ``prefetch`` keeps the next tile's HBM loads in flight in registers while
     the current tile computes; ``waves`` caps the registers at that many
-    waves per SIMD (None: the compiler's choice)."""
+    waves per SIMD (None: the compiler's choice).  ``readout``: per bit
+    position p, the mask whose parity is the measured bit (the measuring pass
+    then writes n + 1 readout partials per slot, qs_readout_finish); None
+    measures the bits themselves (K + 1 partials per slot by local bit,
+    qs_fused_measure_finish)."""
     K = R + TB
     T, NR = 1 << TB, 1 << R
     multi = n < K
@@ -183,7 +187,7 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
     e(f"{name}(JArgs A) {{")
     e(f"  __shared__ __align__(16) float2 tile[{1 << K}];")
     if measure and TB > 6:
-        e(f"  __shared__ float red[{(T // 64) * (K + 1)}];")
+        e(f"  __shared__ float red[{(T // 64) * ((K + 1) if readout is None else (n + 1))}];")
     e("  const int tid = threadIdx.x;")
     e(f"  const i64 total = A.B << {n};")
     e("  i64 g_tid = 0;")
@@ -230,7 +234,7 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
     e(f"    const i64 block_sample = base >> {n};")
     e(f"    float2 v[{NR}];")
     for s, sub in enumerate(subs):
-        _sub_pass(e, sub, s, len(subs), n, R, TB, multi, gen and s == 0, measure and s == len(subs) - 1)
+        _sub_pass(e, sub, s, len(subs), n, R, TB, multi, gen and s == 0, measure and s == len(subs) - 1, readout)
     e("    if (A.store) {")
     e("      const int tz = pinv(tid);")
     for it in range(NR // 2):
@@ -249,7 +253,7 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
     return "\n".join(o)
 
 
-def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure):
+def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None):
     K = R + TB
     T, NR = 1 << TB, 1 << R
     e(f"    {{  // sub-pass {s}")
@@ -338,66 +342,100 @@ def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure):
             e(f"        v[{j}] = cmul(z, v[{j}]);")
         e("      }")
     if measure:
-        _measure(e, sub, phys, n, R, TB, multi)
+        _measure(e, sub, phys, n, R, TB, multi, readout)
     for j in range(NR):
         e(f"      tile[lb ^ {swj[j]}] = v[{phys[j]}];")
     e("      __syncthreads();")
     e("    }")
 
 
-def _measure(e, sub, phys, n, R, TB, multi):
-    """MeasureProbability partials of the tile (layout of qs_fused_measure_finish)."""
+def _parity(var: str, mask: int) -> str:
+    """Expression of the parity of ``var & mask`` (a 64-bit index)."""
+    if mask & (mask - 1) == 0:
+        return f"(({var} >> {mask.bit_length() - 1}) & 1)"
+    return f"(__popcll({var} & {mask}LL) & 1)"
+
+
+def _measure(e, sub, phys, n, R, TB, multi, readout=None):
+    """MeasureProbability partials of the tile: K + 1 per slot by local bit
+    (layout of qs_fused_measure_finish) or, with ``readout`` masks, n + 1 per
+    slot by bit position (layout of qs_readout_finish)."""
     K = R + TB
     T, NR = 1 << TB, 1 << R
     e("      {")
     e(f"        float p[{NR}]; float tot = 0.f;")
     for j in range(NR):
         e(f"        p[{j}] = v[{phys[j]}].x * v[{phys[j]}].x + v[{phys[j]}].y * v[{phys[j]}].y; tot += p[{j}];")
-    e(f"        float c[{K + 1}];")
-    for r in range(R):
-        terms = " + ".join(f"p[{j}]" for j in range(NR) if not (j >> r) & 1)
-        e(f"        c[{r}] = {terms};")
-    for i in range(TB):
-        e(f"        c[{R + i}] = ((tid >> {i}) & 1) ? 0.f : tot;")
-    e(f"        c[{K}] = tot;")
+    if readout is None:
+        W = K + 1
+        e(f"        float c[{W}];")
+        for r in range(R):
+            terms = " + ".join(f"p[{j}]" for j in range(NR) if not (j >> r) & 1)
+            e(f"        c[{r}] = {terms};")
+        for i in range(TB):
+            e(f"        c[{R + i}] = ((tid >> {i}) & 1) ? 0.f : tot;")
+        e(f"        c[{K}] = tot;")
+        dst = [sub.P[q] if q < R else (sub.G[q - R] if q < K else K) for q in range(W)]
+    else:
+        # measured bit p = parity(a_p & x): the register part of a_p selects
+        # which of the thread's amplitudes count, the rest (thread and tile
+        # bits) flips the thread's share to its complement
+        W = n + 1
+        regpos = 0
+        for r in range(R):
+            regpos |= 1 << sub.GP[r]
+        e(f"        float c[{W}];")
+        sums = {}
+        for pos in range(n):
+            a = int(readout[pos])
+            mr = sum(1 << r for r in range(R) if (a >> sub.GP[r]) & 1)
+            rest = a & ~regpos
+            if mr and mr not in sums:
+                terms = " + ".join(f"p[{j}]" for j in range(NR) if bin(j & mr).count("1") % 2 == 0)
+                sums[mr] = f"s{mr}"
+                e(f"        const float s{mr} = {terms};")
+            s0 = sums[mr] if mr else "tot"
+            if rest:
+                e(f"        c[{pos}] = {_parity('gbase', rest)} ? tot - {s0} : {s0};")
+            else:
+                e(f"        c[{pos}] = {s0};")
+        e(f"        c[{n}] = tot;")
+        dst = list(range(W))
     seglog = (n - R) if multi else TB
     wlog = min(seglog, 6)
-    e(f"        for (int q = 0; q <= {K}; ++q) {{ float x = c[q];")
+    e(f"        for (int q = 0; q < {W}; ++q) {{ float x = c[q];")
     e(f"          for (int off = 1; off < {1 << wlog}; off <<= 1) x += __shfl_xor(x, off, 64); c[q] = x; }}")
     spt_log = (K - n) if multi else 0
-    e(f"        float *mp = A.mpart + (t << {spt_log}) * {K + 1};")
+    e(f"        float *mp = A.mpart + (t << {spt_log}) * {W};")
     if seglog <= 6:
         e(f"        if ((tid & {(1 << seglog) - 1}) == 0) {{")
-        e(f"          float *ob = mp + (tid >> {seglog}) * {K + 1};")
-        for r in range(R):
-            e(f"          ob[{sub.P[r]}] = c[{r}];")
-        for i in range(TB):
-            e(f"          ob[{sub.G[i]}] = c[{R + i}];")
-        e(f"          ob[{K}] = c[{K}];")
+        e(f"          float *ob = mp + (tid >> {seglog}) * {W};")
+        for q in range(W):
+            e(f"          ob[{dst[q]}] = c[{q}];")
         e("        }")
     else:
         wps = 1 << (seglog - 6)
         nsamp = (T // 64) // wps
-        dst = [sub.P[q] if q < R else (sub.G[q - R] if q < K else K) for q in range(K + 1)]
         e("        if ((tid & 63) == 0) {")
-        e(f"          for (int q = 0; q <= {K}; ++q) red[(tid >> 6) * {K + 1} + q] = c[q];")
+        e(f"          for (int q = 0; q < {W}; ++q) red[(tid >> 6) * {W} + q] = c[q];")
         e("        }")
         e("        __syncthreads();")
-        e(f"        if (tid < {nsamp * (K + 1)}) {{")
-        e(f"          const int sm = tid / {K + 1}, q = tid % {K + 1};")
+        e(f"        if (tid < {nsamp * W}) {{")
+        e(f"          const int sm = tid / {W}, q = tid % {W};")
         e("          float x = 0.f;")
-        e(f"          for (int k = 0; k < {wps}; ++k) x += red[(sm * {wps} + k) * {K + 1} + q];")
-        e(f"          const int dst[{K + 1}] = {{{', '.join(str(d) for d in dst)}}};")
-        e(f"          mp[sm * {K + 1} + dst[q]] = x;")
+        e(f"          for (int k = 0; k < {wps}; ++k) x += red[(sm * {wps} + k) * {W} + q];")
+        e(f"          const int dst[{W}] = {{{', '.join(str(d) for d in dst)}}};")
+        e(f"          mp[sm * {W} + dst[q]] = x;")
         e("        }")
     e("      }")
 
 
 def adjoint_source(name: str, words, n: int, R: int, TB: int, lpos, seed: bool,
-                   prefetch: bool = False, waves: int = None) -> str:
+                   prefetch: bool = False, waves: int = None, readout=None) -> str:
     """HIP source of one specialised adjoint pass kernel (the adjoint tile of
     qs_adjoint_pass with the pass program compiled in).  ``seed``: the first
-    step of the sweep, whose co-state is built from psi and w = dL/dP."""
+    step of the sweep, whose co-state is built from psi and w = dL/dP, with
+    the measured bit at position p the parity of ``readout[p]`` (None: bit p)."""
     K = R + TB
     T, NR = 1 << TB, 1 << R
     multi = n < K
@@ -461,10 +499,20 @@ def adjoint_source(name: str, words, n: int, R: int, TB: int, lpos, seed: bool,
             e(f"      const i64 gi = base | g_tid | {goff[it]}LL;")
             e(f"      const i64 smp = {'gi >> ' + str(n) if multi else 'block_sample'};")
             e(f"      const float *wb = A.seed_w + (gi < total ? smp : 0) * {n};")
-            e("      float w0 = 0.f;")
-            for p in range(1, n):
-                e(f"      w0 += ((gi >> {p}) & 1) ? 0.f : wb[{n - 1 - p}];")
-            e(f"      const float w1 = 2.f * w0; w0 = 2.f * (w0 + wb[{n - 1}]);")
+            # gi is even: wc collects the weights whose measured bit is 0 at both gi
+            # and gi + 1, wa (wz) those measured 0 only at gi (gi + 1)
+            e("      float wc = 0.f, wa = 0.f, wz = 0.f;")
+            masks = [1 << p for p in range(n)] if readout is None else [int(x) for x in readout]
+            for p in list(range(1, n)) + [0]:
+                a = masks[p]
+                if not a & 1:
+                    e(f"      wc += {_parity('gi', a)} ? 0.f : wb[{n - 1 - p}];")
+                elif a == 1:
+                    e(f"      wa += wb[{n - 1 - p}];")
+                else:
+                    e(f"      {{ const bool o = {_parity('gi', a ^ 1)}; wa += o ? 0.f : wb[{n - 1 - p}]; "
+                      f"wz += o ? wb[{n - 1 - p}] : 0.f; }}")
+            e("      const float w0 = 2.f * (wc + wa), w1 = 2.f * (wc + wz);")
             e("      const float4 b = make_float4(w0 * a.x, w0 * a.y, w1 * a.z, w1 * a.w);")
         else:
             e(f"      const float4 b = pb[{it}];")
@@ -714,7 +762,7 @@ class ProgramKernels:
     """The specialised kernels of every pass of one encoded forward program."""
 
     def __init__(self, passes, n: int, R: int, TB: int, gen_first: bool, measure: bool,
-                 prefetch: bool = False, waves: int = None):
+                 prefetch: bool = False, waves: int = None, readout=None):
         self.n, self.R, self.TB = n, R, TB
         self.block = 1 << TB
         npass = len(passes)
@@ -723,7 +771,7 @@ class ProgramKernels:
             words = w.cpu().numpy() if hasattr(w, "cpu") else w
             nm = f"{KERNEL}_{i}"
             self.sources.append(pass_source(nm, words, n, R, TB, lpos, gen_first and i == 0,
-                                            measure and i == npass - 1, prefetch, waves))
+                                            measure and i == npass - 1, prefetch, waves, readout))
             self.names.append(nm)
         self.funcs = kernels_for(self.sources, self.names)
         self.grid_cap = [None] * len(self.funcs)
@@ -744,13 +792,14 @@ class AdjointKernels:
     """The specialised kernels of every step of one encoded adjoint sweep."""
 
     def __init__(self, passes, n: int, R: int, TB: int, seed_first: bool, prefetch: bool = False,
-                 waves: int = None):
+                 waves: int = None, readout=None):
         self.block = 1 << TB
         self.sources, self.names = [], []
         for i, (w, _nsub, lpos) in enumerate(passes):
             words = w.cpu().numpy() if hasattr(w, "cpu") else w
             nm = f"qs_adj_{i}"
-            self.sources.append(adjoint_source(nm, words, n, R, TB, lpos, seed_first and i == 0, prefetch, waves))
+            self.sources.append(adjoint_source(nm, words, n, R, TB, lpos, seed_first and i == 0, prefetch, waves,
+                                               readout))
             self.names.append(nm)
         self.funcs = kernels_for(self.sources, self.names)
         self.grid_cap = [None] * len(self.funcs)

@@ -424,6 +424,22 @@ __global__ void __launch_bounds__(256) k_measure_finish_items(const FinishArgs *
   for (int64_t b = 0; b < F.B; ++b) measure_finish_block(F, b, q, red);
 }
 
+// readout partials: one block per (sample, qubit), double accumulation over
+// the sample's slots (n + 1 floats each, entry n - 1 - q for qubit q)
+__global__ void __launch_bounds__(256) k_readout_finish(float *out, const float *mpart, int n, int64_t sps) {
+  __shared__ double red[4];
+  const int64_t b = blockIdx.x;
+  const int q = blockIdx.y;
+  const int pos = n - 1 - q;
+  double acc = 0.0;
+  for (int64_t s = threadIdx.x; s < sps; s += 256) acc += mpart[(b * sps + s) * (n + 1) + pos];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[b * n + q] = (float)(red[0] + red[1] + red[2] + red[3]);
+}
+
 // (B, n, 2) product-state factors of an AngleEmbedding (or |0> for unembedded bits)
 struct FactorArgs {
   int32_t slot_of_bit[QS_MAX_QUBITS];
@@ -766,6 +782,23 @@ int qs_fused_measure_finish(float *out, const float *mpart, int64_t B, int n, in
   if (B > 0x7fffffff) return QS_EUNSUPPORTED;
   hipLaunchKernelGGL(k_measure_finish, dim3((unsigned)B, (unsigned)n), dim3(256), 0,
                      (hipStream_t)stream, F);
+  return launch_status();
+}
+
+int64_t qs_readout_work_bytes(int64_t B, int n, int R, int TB) {
+  if (B < 1 || n < 1 || n > QS_MAX_QUBITS || !qs_fused_supported(R, TB)) return -1;
+  const int K = R + TB;
+  const int64_t spt = n >= K ? 1 : (int64_t(1) << (K - n));
+  return fused_tiles(B, n, K) * spt * (n + 1) * (int64_t)sizeof(float);
+}
+
+int qs_readout_finish(float *out, const float *mpart, int64_t B, int n, int R, int TB, void *stream) {
+  if (B < 1 || n < 1 || n > QS_MAX_QUBITS || !out || !mpart || !qs_fused_supported(R, TB)) return QS_EINVAL;
+  if (B > 0x7fffffff) return QS_EUNSUPPORTED;
+  const int K = R + TB;
+  const int64_t sps = n >= K ? (int64_t(1) << (n - K)) : 1;
+  hipLaunchKernelGGL(k_readout_finish, dim3((unsigned)B, (unsigned)n), dim3(256), 0, (hipStream_t)stream, out,
+                     mpart, n, sps);
   return launch_status();
 }
 

@@ -136,3 +136,51 @@ def test_specialised_adjoint_vs_fp64(device, specialise, n, B, depth):
           f"vs interpreted {np.abs(gx1 - gx0).max():.2e} {np.abs(gt1 - gt0).max():.2e}")
     assert ex < 1e-4 and et < 1e-4
     assert np.abs(gx1 - gx0).max() < 1e-5 and np.abs(gt1 - gt0).max() < 1e-5
+
+
+@pytest.mark.parametrize("n,B", [(7, 4), (15, 3)])
+def test_specialised_folded_readout_and_gradients(device, specialise, n, B):
+    """A random CNOT network ends the circuit: the compiled kernels fold it
+    into parity readouts (forward) and parity co-state seeds (adjoint), with
+    masks that include bit 0 (the pair bit of the seed's float4 loads)."""
+    import math
+
+    rng = np.random.default_rng(40 + n)
+    body = []
+    for q in range(n):
+        body += [("ry", q, float(rng.uniform(0, 2 * math.pi))), ("rz", q, float(rng.uniform(0, 2 * math.pi)))]
+    body += [("cnot", (q, q + 1)) for q in range(n - 1)]
+    body += [("rx", q, float(rng.uniform(0, 2 * math.pi))) for q in range(n)]
+    body += [("cnot", (n - 1, 0)), ("cnot", (0, n - 1)), ("cnot", (2, n - 1))]
+    for _ in range(8):
+        c, t = rng.choice(n, 2, replace=False)
+        body.append(("cnot", (int(c), int(t))))
+    x = torch.from_numpy(rng.uniform(0, math.pi, (B, n)).astype(np.float32))
+    w = workloads._assemble(f"fold_n{n}", n, B, x, body)
+    from qandle_amd._engine import fused
+
+    prog = fused.program_from_spec(n, w.gates, w.embed)
+    _, masks = fused.readout_fold(prog)
+    assert any(m & 1 and m != 1 for m in masks)
+    circ = qandle.Circuit(w.layers, num_qubits=n).to(device)
+    thetas = [m for m in circ.circuit.layers if isinstance(m, qandle.operators.BuiltParametrizedOperator)]
+    wt = torch.from_numpy(rng.standard_normal((B, n)).astype(np.float32))
+
+    def run():
+        xx = w.inputs["x"].to(device).requires_grad_(True)
+        for m in thetas:
+            m.theta.grad = None
+        P = circ(x=xx)
+        (P * wt.to(device)).sum().backward()
+        return P.detach().cpu().numpy(), xx.grad.cpu().numpy(), np.array([m.theta.grad.item() for m in thetas])
+
+    (P0, gx0, gt0), (P1, gx1, gt1) = both(run)
+    ops = w.oracle_ops()
+    ref = fp64.measure_prob(fp64.simulate(n, ops), n)
+    adj = fp64.adjoint_grads(n, ops, wt.double().numpy())
+    ot = np.array([a for a, op in zip(adj[1:], ops[1:]) if op[0] != "cnot"])
+    print(f"folded readout n={n}: |P err| {np.abs(P1 - ref).max():.2e} |dx err| {np.abs(gx1 - adj[0]).max():.2e} "
+          f"|dtheta err| {np.abs(gt1 - ot).max():.2e}")
+    assert np.abs(P1 - ref).max() < ATOL and np.abs(P1 - P0).max() < 1e-6
+    assert np.abs(gx1 - adj[0]).max() < 1e-4 and np.abs(gt1 - ot).max() < 1e-4
+    assert np.abs(gx1 - gx0).max() < 1e-5 and np.abs(gt1 - gt0).max() < 1e-5
