@@ -112,6 +112,12 @@ int qs_pauli_im(float *out, const float *lam, const float *psi, int64_t B,
 /* 1 when a (register bits R, thread bits TB) kernel variant is compiled in. */
 int qs_fused_supported(int R, int TB);
 
+/* 1 when the measurement and gradient partial layouts of a (R, TB) tile are
+ * defined (the run-time specialised kernels also use shapes that no compiled
+ * variant has): 1 <= R <= 5, 6 <= TB <= 10, R + TB <= 16.  The work-bytes and
+ * finish entry points below accept these shapes. */
+int qs_layout_supported(int R, int TB);
+
 /* Product-state factors efac[b, pos, v] (float2) of an AngleEmbedding over bit
  * positions bits[0..d) (host) with angles x (B, d), row stride x_stride;
  * positions that are not embedded get (1, 0) / (0, 0) (|0>).  d = 0 gives

@@ -140,6 +140,16 @@ struct JAdj {
   float *psi; float *lam; const int *utab; const int *qtab; const int *cs; float *gpart; const float *seed_w;
   i64 B; i64 NS; i64 ntiles;
 };
+// vector-ALU form with the unitary read through the scalar path (uniform tables)
+__device__ __forceinline__ void valu_pair_s(float2 &x, float2 &y, const int *up) {
+  cint *u = as_const(up);
+  const float a = __int_as_float(u[0]), b = __int_as_float(u[1]), c = __int_as_float(u[2]),
+              d = __int_as_float(u[3]), e = __int_as_float(u[4]), f = __int_as_float(u[5]),
+              g = __int_as_float(u[6]), h = __int_as_float(u[7]);
+  const float2 p = x, q = y;
+  x = make_float2(a * p.x - b * p.y + c * q.x - d * q.y, a * p.y + b * p.x + c * q.y + d * q.x);
+  y = make_float2(e * p.x - f * p.y + g * q.x - h * q.y, e * p.y + f * p.x + g * q.y + h * q.x);
+}
 // vector-ALU form (per-sample unitaries that may differ inside a 4-lane block)
 __device__ __forceinline__ void valu_pair(float2 &x, float2 &y, const int *u) {
   const float a = __int_as_float(u[0]), b = __int_as_float(u[1]), c = __int_as_float(u[2]),
@@ -163,7 +173,7 @@ def _swz(l: int) -> int:
 
 
 def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, measure: bool,
-                prefetch: bool = False, waves: int = None, readout=None) -> str:
+                prefetch: bool = False, waves: int = None, readout=None, valu_pairs: int = 0) -> str:
     """HIP source of one specialised forward pass kernel (extern "C" ``name``).
 
     ``prefetch`` keeps the next tile's HBM loads in flight in registers while
@@ -172,7 +182,9 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
     position p, the mask whose parity is the measured bit (the measuring pass
     then writes n + 1 readout partials per slot, qs_readout_finish); None
     measures the bits themselves (K + 1 partials per slot by local bit,
-    qs_fused_measure_finish)."""
+    qs_fused_measure_finish).  ``valu_pairs``: of the amplitude pairs a
+    thread updates per uniform fused unitary, that many run on the vector ALU
+    and the rest on the matrix cores (the two units then share the work)."""
     K = R + TB
     T, NR = 1 << TB, 1 << R
     multi = n < K
@@ -234,7 +246,8 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
     e(f"    const i64 block_sample = base >> {n};")
     e(f"    float2 v[{NR}];")
     for s, sub in enumerate(subs):
-        _sub_pass(e, sub, s, len(subs), n, R, TB, multi, gen and s == 0, measure and s == len(subs) - 1, readout)
+        _sub_pass(e, sub, s, len(subs), n, R, TB, multi, gen and s == 0, measure and s == len(subs) - 1, readout,
+                  valu_pairs)
     e("    if (A.store) {")
     e("      const int tz = pinv(tid);")
     for it in range(NR // 2):
@@ -253,7 +266,7 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
     return "\n".join(o)
 
 
-def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None):
+def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu_pairs=0):
     K = R + TB
     T, NR = 1 << TB, 1 << R
     e(f"    {{  // sub-pass {s}")
@@ -309,9 +322,14 @@ def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None):
                         e(f"        valu_pair(v[{phys[j]}], v[{phys[j | (1 << r)]}], u);")
             else:
                 e(f"        float m[4]; coeffs(ut + {off}, m);")
+                k = 0
                 for j in range(NR):
                     if not (j >> r) & 1:
-                        e(f"        mfma_pair(v[{phys[j]}], v[{phys[j | (1 << r)]}], m);")
+                        if k < valu_pairs:
+                            e(f"        valu_pair_s(v[{phys[j]}], v[{phys[j | (1 << r)]}], ut + {off});")
+                        else:
+                            e(f"        mfma_pair(v[{phys[j]}], v[{phys[j | (1 << r)]}], m);")
+                        k += 1
             e("      }")
         for uniform, tgt, ctl in cnots:
             if not uniform:
@@ -762,26 +780,31 @@ class ProgramKernels:
     """The specialised kernels of every pass of one encoded forward program."""
 
     def __init__(self, passes, n: int, R: int, TB: int, gen_first: bool, measure: bool,
-                 prefetch: bool = False, waves: int = None, readout=None):
+                 prefetch: bool = False, waves: int = None, readout=None, valu_pairs: int = 0,
+                 grid: str = "full"):
         self.n, self.R, self.TB = n, R, TB
         self.block = 1 << TB
+        self.full_grid = grid == "full"
         npass = len(passes)
         self.sources, self.names = [], []
         for i, (w, _nsub, lpos) in enumerate(passes):
             words = w.cpu().numpy() if hasattr(w, "cpu") else w
             nm = f"{KERNEL}_{i}"
             self.sources.append(pass_source(nm, words, n, R, TB, lpos, gen_first and i == 0,
-                                            measure and i == npass - 1, prefetch, waves, readout))
+                                            measure and i == npass - 1, prefetch, waves, readout, valu_pairs))
             self.names.append(nm)
         self.funcs = kernels_for(self.sources, self.names)
         self.grid_cap = [None] * len(self.funcs)
 
     def launch(self, i: int, state, src, utab, cs, efac, mpart, B: int, ntiles: int, store: int, stream):
         lib = L.load()
-        if self.grid_cap[i] is None:
-            g = lib.qs_jit_resident_blocks(self.funcs[i], self.block)
-            self.grid_cap[i] = int(g) if g and g > 0 else 2048
-        grid = max(1, min(ntiles, self.grid_cap[i]))
+        if self.full_grid:
+            grid = max(1, min(ntiles, 0x7fffffff))
+        else:
+            if self.grid_cap[i] is None:
+                g = lib.qs_jit_resident_blocks(self.funcs[i], self.block)
+                self.grid_cap[i] = int(g) if g and g > 0 else 2048
+            grid = max(1, min(ntiles, self.grid_cap[i]))
         args = struct.pack(ARGS_FMT, state or 0, src or 0, utab or 0, cs or 0, efac or 0, mpart or 0,
                            B, ntiles, store, 0)
         buf = ctypes.create_string_buffer(args, len(args))
@@ -792,8 +815,9 @@ class AdjointKernels:
     """The specialised kernels of every step of one encoded adjoint sweep."""
 
     def __init__(self, passes, n: int, R: int, TB: int, seed_first: bool, prefetch: bool = False,
-                 waves: int = None, readout=None):
+                 waves: int = None, readout=None, grid: str = "full"):
         self.block = 1 << TB
+        self.full_grid = grid == "full"
         self.sources, self.names = [], []
         for i, (w, _nsub, lpos) in enumerate(passes):
             words = w.cpu().numpy() if hasattr(w, "cpu") else w
@@ -806,10 +830,13 @@ class AdjointKernels:
 
     def launch(self, i: int, psi, lam, utab, qtab, cs, gpart, seed_w, B: int, NS: int, ntiles: int, stream):
         lib = L.load()
-        if self.grid_cap[i] is None:
-            g = lib.qs_jit_resident_blocks(self.funcs[i], self.block)
-            self.grid_cap[i] = int(g) if g and g > 0 else 2048
-        grid = max(1, min(ntiles, self.grid_cap[i]))
+        if self.full_grid:
+            grid = max(1, min(ntiles, 0x7fffffff))
+        else:
+            if self.grid_cap[i] is None:
+                g = lib.qs_jit_resident_blocks(self.funcs[i], self.block)
+                self.grid_cap[i] = int(g) if g and g > 0 else 2048
+            grid = max(1, min(ntiles, self.grid_cap[i]))
         args = struct.pack(ADJ_FMT, psi, lam, utab or 0, qtab or 0, cs or 0, gpart, seed_w or 0, B, NS, ntiles)
         buf = ctypes.create_string_buffer(args, len(args))
         L.check(lib.qs_jit_launch(self.funcs[i], grid, self.block, buf, len(args), stream), "qs_jit_launch")

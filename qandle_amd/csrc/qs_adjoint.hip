@@ -307,7 +307,7 @@ static int launch_adj(const AdjArgs &A, bool multi, int64_t ntiles, hipStream_t 
 extern "C" {
 
 int64_t qs_adjoint_work_bytes(int64_t B, int n, int R, int TB, int rows) {
-  if (B < 1 || n < 1 || n > QS_MAX_QUBITS || rows < 0 || !qs_fused_supported(R, TB) || R > n)
+  if (B < 1 || n < 1 || n > QS_MAX_QUBITS || rows < 0 || !qs_layout_supported(R, TB) || R > n)
     return -1;
   int64_t ntiles, NS, sps;
   adj_geometry(B, n, R, TB, &ntiles, &NS, &sps);
@@ -426,7 +426,7 @@ int qs_items_adjoint(const void *descs, const int32_t *items, int nitems, int R,
 
 static int fill_adj_finish(AdjFinishArgs &F, float *out, float *tot, const float *gpart, int rows,
                            int64_t B, int n, int R, int TB) {
-  if (B < 1 || n < 1 || n > QS_MAX_QUBITS || rows < 0 || !qs_fused_supported(R, TB) || R > n)
+  if (B < 1 || n < 1 || n > QS_MAX_QUBITS || rows < 0 || !qs_layout_supported(R, TB) || R > n)
     return QS_EINVAL;
   if (rows > 0 && (!out || !tot || !gpart)) return QS_EINVAL;
   int64_t ntiles;

@@ -606,8 +606,12 @@ int qs_fused_supported(int R, int TB) {
   return (TB == 8 && R >= 1 && R <= 5) || (TB == 7 && (R == 5 || R == 4)) || (TB == 6 && R == 5);
 }
 
+int qs_layout_supported(int R, int TB) {
+  return R >= 1 && R <= 5 && TB >= 6 && TB <= 10 && R + TB <= QS_MAX_LOCAL_BITS;
+}
+
 int64_t qs_fused_measure_work_bytes(int64_t B, int n, int R, int TB) {
-  if (B < 1 || n < 1 || n > QS_MAX_QUBITS || !qs_fused_supported(R, TB)) return -1;
+  if (B < 1 || n < 1 || n > QS_MAX_QUBITS || !qs_layout_supported(R, TB)) return -1;
   const int K = R + TB;
   const int64_t spt = n >= K ? 1 : (int64_t(1) << (K - n));
   return fused_tiles(B, n, K) * spt * (K + 1) * (int64_t)sizeof(float);
@@ -747,7 +751,7 @@ int qs_items_pass(const void *descs, const int32_t *items, int nitems, int R, in
 
 static int fill_finish_args(FinishArgs &F, float *out, const float *mpart, int64_t B, int n, int R,
                             int TB, const int32_t *lpos) {
-  if (B < 1 || n < 1 || n > QS_MAX_QUBITS || !out || !mpart || !lpos || !qs_fused_supported(R, TB))
+  if (B < 1 || n < 1 || n > QS_MAX_QUBITS || !out || !mpart || !lpos || !qs_layout_supported(R, TB))
     return QS_EINVAL;
   const int K = R + TB;
   F.out = out;
@@ -786,14 +790,14 @@ int qs_fused_measure_finish(float *out, const float *mpart, int64_t B, int n, in
 }
 
 int64_t qs_readout_work_bytes(int64_t B, int n, int R, int TB) {
-  if (B < 1 || n < 1 || n > QS_MAX_QUBITS || !qs_fused_supported(R, TB)) return -1;
+  if (B < 1 || n < 1 || n > QS_MAX_QUBITS || !qs_layout_supported(R, TB)) return -1;
   const int K = R + TB;
   const int64_t spt = n >= K ? 1 : (int64_t(1) << (K - n));
   return fused_tiles(B, n, K) * spt * (n + 1) * (int64_t)sizeof(float);
 }
 
 int qs_readout_finish(float *out, const float *mpart, int64_t B, int n, int R, int TB, void *stream) {
-  if (B < 1 || n < 1 || n > QS_MAX_QUBITS || !out || !mpart || !qs_fused_supported(R, TB)) return QS_EINVAL;
+  if (B < 1 || n < 1 || n > QS_MAX_QUBITS || !out || !mpart || !qs_layout_supported(R, TB)) return QS_EINVAL;
   if (B > 0x7fffffff) return QS_EUNSUPPORTED;
   const int K = R + TB;
   const int64_t sps = n >= K ? (int64_t(1) << (n - K)) : 1;
