@@ -97,6 +97,66 @@ int qs_pauli_im(float *out, const float *lam, const float *psi, int64_t B,
                 int n, const int32_t *bits, int d, int pauli, void *work,
                 int64_t work_bytes, void *stream);
 
+/* qs_pauli_im restricted to the amplitudes whose bits cmask are all 1 (the
+ * generator P (x) |1..1><1..1| of a controlled rotation; cmask must not hold
+ * a bit of `bits`).  Gradient of a Controlled rotation
+ * (src/qandle/operators.py:488-506) in the plain path. */
+int qs_pauli_im_controlled(float *out, const float *lam, const float *psi,
+                           int64_t B, int n, const int32_t *bits, int d,
+                           int pauli, int64_t cmask, void *work,
+                           int64_t work_bytes, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Remaining operators, plain per-gate path.  `cmask` is a set of bit
+ * positions (bit p of the mask = position p) that must all be 1 for the gate
+ * to act; 0 = uncontrolled.  A 2x2 unitary `u` is 8 floats, row-major complex
+ * (u00, u01, u10, u11) in the column convention psi' = U psi, i.e. the
+ * transpose of the reference's `state @ M` block.
+ * ------------------------------------------------------------------------- */
+
+/* psi' = U psi on bit position `bit` where every bit of cmask is set (out may
+ * equal in).  u_stride 0: one U for the batch; 1: one U per sample (8 floats
+ * each).  dagger != 0 applies U^dagger.  Replaces BuiltU.forward
+ * (src/qandle/operators.py:125-126), BuiltControlled.forward (:488-506) and
+ * BuiltInvert.forward (:445-447) for one-qubit targets. */
+int qs_unitary_1q(float *out, const float *in, int64_t B, int n, int bit,
+                  const float *u, int64_t u_stride, int64_t cmask, int dagger,
+                  void *stream);
+
+/* u[8k..8k+8) = R_kind(phi_k) from cs (cos, sin of phi_k / 2), the matrices of
+ * src/qandle/operators.py:368-395 in the column convention. */
+int qs_rot_unitaries(float *u, const float *cs, int64_t count, int kind,
+                     void *stream);
+
+/* Multi-controlled X: out[i] = in[i ^ (1 << tbit)] where every bit of cmask is
+ * set in i, else in[i] (out != in).  CNOT under a Controlled
+ * (src/qandle/operators.py:488-506, 557-558); exact permutation. */
+int qs_mcx(float *out, const float *in, int64_t B, int n, int tbit,
+           int64_t cmask, void *stream);
+
+/* out[i] = -in[i] where every bit of mask is set in i (out may equal in).
+ * CZ is mask = {control, target}; replaces BuiltCZ.forward
+ * (src/qandle/operators.py:581-591). */
+int qs_phase_flip(float *out, const float *in, int64_t B, int n, int64_t mask,
+                  void *stream);
+
+/* out[i] = in[i with bits abit and bbit exchanged] where every bit of cmask is
+ * set in i (out != in).  Replaces BuiltSWAP.forward
+ * (src/qandle/operators.py:662-674). */
+int qs_swap(float *out, const float *in, int64_t B, int n, int abit, int bbit,
+            int64_t cmask, void *stream);
+
+/* Reset of bit position `bit`, per amplitude pair (a0: bit 0, a1: bit 1):
+ * a0' = a0 * sqrt(|a0|^2 + |a1|^2) / (|a0| + 1e-7), a1' = 0 (out may equal
+ * in).  Replaces BuiltReset.forward (src/qandle/operators.py:612-623). */
+int qs_reset(float *out, const float *in, int64_t B, int n, int bit,
+             void *stream);
+
+/* Gradient of qs_reset for a real loss: gin from g = dL/dRe + i dL/dIm of
+ * the output and the input `in` of the forward (gin != in). */
+int qs_reset_backward(float *gin, const float *g, const float *in, int64_t B,
+                      int n, int bit, void *stream);
+
 /* ---------------------------------------------------------------------------
  * Fused pass path.  A pass loads a tile of 2^K amplitudes (K = R + TB local
  * bit positions, lpos[] ascending with lpos[0] = 0) into LDS once, applies a

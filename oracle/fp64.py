@@ -22,6 +22,21 @@ product:
     ("embed", rot, qubits, X)   rot in "xyz", X: (B, d) or (d,) angles
     ("rx" | "ry" | "rz", q, phi)  phi: float (shared) or (B,) array (per sample)
     ("cnot", c, t)
+    ("u", q, M)                  custom gate, M the reference's 2x2 ``matrix``
+    ("cz", c, t) | ("swap", a, b) | ("reset", q)
+    ("ctrl", c, op)              op (one of the above gate tuples) where qubit c is 1
+    ("inv", op)                  inverse of a gate tuple
+
+The remaining operators (``src/qandle/operators.py:72-147, 416-699``):
+
+* U(q, M): ``state @ kron(.., M at q, ..)`` (``:87-126``), i.e. psi' = M^T psi.
+* Invert(T): ``state @ inv(T.to_matrix())`` (``:445-447``).
+* Controlled(c, T): T's matrix on the c = 1 half, the c = 0 half unchanged
+  (``:488-506``); its own ``to_matrix`` is T's (``:514-515``).
+* CZ: -1 on the amplitudes with bits c and t set (``:581-591``).
+* SWAP: exchange of bits a and b (``:662-674``).
+* Reset(q): per amplitude pair (a0: q = 0, a1: q = 1),
+  a0' = a0 |(a0, a1)| / (|a0| + 1e-7), a1' = 0 (``:612-623``).
 """
 from __future__ import annotations
 
@@ -82,6 +97,51 @@ def apply_cnot(psi: np.ndarray, n: int, c: int, t: int) -> np.ndarray:
     cm, tm = 1 << bit(n, c), 1 << bit(n, t)
     src = np.where(idx & cm, idx ^ tm, idx)
     return psi[:, src]
+
+
+def apply_u(psi: np.ndarray, n: int, q: int, M: np.ndarray, ctrl=()) -> np.ndarray:
+    """U(q, M) (``src/qandle/operators.py:87-126``): psi' = M^T psi on qubit q,
+    only where every qubit in ``ctrl`` is 1 (``:488-506``)."""
+    out = apply_1q(psi, n, q, np.swapaxes(np.asarray(M, dtype=np.complex128), -1, -2))
+    if not ctrl:
+        return out
+    on = _ctrl_mask(n, ctrl)
+    return np.where(on[None, :], out, psi)
+
+
+def _ctrl_mask(n: int, ctrl) -> np.ndarray:
+    idx = np.arange(2 ** n)
+    on = np.ones(2 ** n, dtype=bool)
+    for c in ctrl:
+        on &= ((idx >> bit(n, c)) & 1) == 1
+    return on
+
+
+def apply_cz(psi: np.ndarray, n: int, c: int, t: int, ctrl=()) -> np.ndarray:
+    """CZ (``src/qandle/operators.py:581-591``): -1 where bits c and t are set."""
+    on = _ctrl_mask(n, tuple(ctrl) + (c, t))
+    return np.where(on[None, :], -psi, psi)
+
+
+def apply_swap(psi: np.ndarray, n: int, a: int, b: int, ctrl=()) -> np.ndarray:
+    """SWAP (``src/qandle/operators.py:662-674``): out[i] = psi[i with bits a, b exchanged]."""
+    idx = np.arange(2 ** n)
+    ma, mb = 1 << bit(n, a), 1 << bit(n, b)
+    differ = ((idx & ma) != 0) != ((idx & mb) != 0)
+    src = np.where(differ & _ctrl_mask(n, ctrl), idx ^ (ma | mb), idx)
+    return psi[:, src]
+
+
+def apply_reset(psi: np.ndarray, n: int, q: int) -> np.ndarray:
+    """Reset (``src/qandle/operators.py:612-623``), per pair of amplitudes that
+    differ only in qubit q: a0' = a0 * |(a0, a1)| / (|a0| + 1e-7), a1' = 0."""
+    B = psi.shape[0]
+    v = psi.reshape(B, 2 ** q, 2, 2 ** (n - 1 - q))
+    a0, a1 = v[:, :, 0, :], v[:, :, 1, :]
+    norm = np.sqrt(np.abs(a0) ** 2 + np.abs(a1) ** 2)
+    out = np.zeros_like(v)
+    out[:, :, 0, :] = a0 * norm / (np.abs(a0) + 1e-7)
+    return out.reshape(B, 2 ** n)
 
 
 def embed(X: np.ndarray, n: int, qubits, rot: str) -> np.ndarray:
@@ -146,11 +206,70 @@ def simulate(n: int, ops, psi0: np.ndarray | None = None, record: bool = False):
             psi = apply_1q(psi, n, op[1], U)
         elif kind == "cnot":
             psi = apply_cnot(psi, n, op[1], op[2])
+        elif kind in ("u", "cz", "swap", "reset", "ctrl", "inv"):
+            psi = apply_gate(psi, n, op)
         else:
             raise ValueError(kind)
         if record:
             states.append(psi.copy())
     return (psi, states) if record else psi
+
+
+def gate_matrix_1q(op):
+    """(qubit, reference-convention 2x2 matrix M) of a one-qubit gate tuple
+    (``state @ M`` form), for ``ctrl`` and ``inv``."""
+    kind = op[0]
+    if kind in ("rx", "ry", "rz"):
+        return op[1], np.swapaxes(rot_matrix(kind[1], op[2]), -1, -2)
+    if kind == "u":
+        return op[1], np.asarray(op[2], dtype=np.complex128)
+    return None
+
+
+def apply_gate(psi: np.ndarray, n: int, op, ctrl=()) -> np.ndarray:
+    """One gate tuple, with extra control qubits ``ctrl``."""
+    kind = op[0]
+    if kind in ("rx", "ry", "rz", "u"):
+        q, M = gate_matrix_1q(op)
+        return apply_u(psi, n, q, M, ctrl)
+    if kind == "cnot":
+        return apply_u(psi, n, op[2], np.array([[0, 1], [1, 0]]), tuple(ctrl) + (op[1],))
+    if kind == "cz":
+        return apply_cz(psi, n, op[1], op[2], ctrl)
+    if kind == "swap":
+        return psi.copy() if op[1] == op[2] else apply_swap(psi, n, op[1], op[2], ctrl)
+    if kind == "reset":
+        if ctrl:
+            raise ValueError("Reset gate does not have a matrix representation.")
+        return apply_reset(psi, n, op[1])
+    if kind == "ctrl":
+        # the target's matrix (an inner Controlled contributes only its own target)
+        inner = op[2]
+        while inner[0] == "ctrl":
+            inner = inner[2]
+        return apply_gate(psi, n, inner, tuple(ctrl) + (op[1],))
+    if kind == "inv":
+        return apply_gate(psi, n, inverse_of(op[1]), ctrl)
+    raise ValueError(kind)
+
+
+def inverse_of(op):
+    """Gate tuple of inv(op.to_matrix()) (``src/qandle/operators.py:445-447``)."""
+    kind = op[0]
+    if kind in ("rx", "ry", "rz"):
+        return (kind, op[1], -np.asarray(op[2]))
+    if kind == "u":
+        return ("u", op[1], np.linalg.inv(np.asarray(op[2], dtype=np.complex128)))
+    if kind in ("cnot", "cz", "swap"):
+        return op
+    if kind == "ctrl":  # to_matrix of a Controlled is its target's
+        inner = op[2]
+        while inner[0] == "ctrl":
+            inner = inner[2]
+        return inverse_of(inner)
+    if kind == "inv":
+        return op[1]
+    raise ValueError("Reset gate does not have a matrix representation.")
 
 
 # ----------------------------------------------------------------------------

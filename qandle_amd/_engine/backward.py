@@ -69,7 +69,7 @@ class FusedCircuitFn(torch.autograd.Function):
                 ckpts[i] = psi.clone()
 
         psi, probs = fused.execute(prog, inp, cs, efac, R, TB, cm=cm, want_state=True, checkpoints=keep,
-                                   specialise=jit_on)
+                                   specialise=jit_on, fold=fold)
         ctx.prog, ctx.R, ctx.TB, ctx.cm, ctx.B, ctx.names = prog, R, TB, cm, B, names
         ctx.jit, ctx.fold = jit_on, fold
         ctx.ck_index = sorted(ckpts)

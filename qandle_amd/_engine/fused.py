@@ -30,6 +30,7 @@ class Slot:
     module: typing.Any = None
     name: str = ""
     value: float = 0.0
+    sign: float = 1.0             # -1: the rotation is inverted (Invert of the module)
 
 
 @dataclasses.dataclass
@@ -57,8 +58,24 @@ def _lower_into(layers, n, mapping, prog_gates, slots, state):
             slot = Slot("named", name=mod.name) if mod.named else Slot("param", module=mod)
             slots.append(slot)
             prog_gates.append(PL.Gate(mod.KIND, n - 1 - q, -1, len(slots) - 1))
-        elif isinstance(mod, operators.BuiltCNOT):
-            prog_gates.append(PL.Gate(PL.CNOT, n - 1 - mapping(mod.t), n - 1 - mapping(mod.c)))
+        elif isinstance(mod, operators.BuiltCNOT) or (
+                isinstance(mod, operators.BuiltInvert) and isinstance(mod.target, operators.BuiltCNOT)):
+            # CNOT is its own inverse
+            g = mod if isinstance(mod, operators.BuiltCNOT) else mod.target
+            prog_gates.append(PL.Gate(PL.CNOT, n - 1 - mapping(g.t), n - 1 - mapping(g.c)))
+        elif isinstance(mod, operators.BuiltSWAP) or (
+                isinstance(mod, operators.BuiltInvert) and isinstance(mod.target, operators.BuiltSWAP)):
+            # SWAP(a, b) = CNOT(a, b) CNOT(b, a) CNOT(a, b), an exact permutation (its own inverse)
+            g = mod if isinstance(mod, operators.BuiltSWAP) else mod.target
+            if g.a != g.b:
+                pa, pb = n - 1 - mapping(g.a), n - 1 - mapping(g.b)
+                prog_gates += [PL.Gate(PL.CNOT, pb, pa), PL.Gate(PL.CNOT, pa, pb), PL.Gate(PL.CNOT, pb, pa)]
+        elif (isinstance(mod, operators.BuiltInvert) and isinstance(mod.target, operators.BuiltParametrizedOperator)
+              and not mod.target.named):
+            # R_P(phi)^-1 = R_P(-phi) (the reference inverts target.to_matrix(), src/qandle/operators.py:445-447)
+            t = mod.target
+            slots.append(Slot("param", module=t, sign=-1.0))
+            prog_gates.append(PL.Gate(t.KIND, n - 1 - mapping(t.qubit), -1, len(slots) - 1))
         elif isinstance(mod, embeddings.AngleEmbeddingBuilt):
             # the embedding discards the incoming state: everything before it is dead
             prog_gates.clear()
@@ -375,17 +392,24 @@ def param_angles(prog: Program, device) -> torch.Tensor:
         order.append(values)
     flat = [k for ks in order for k in ks]
     phi = pieces[0] if len(pieces) == 1 else torch.cat(pieces)
-    if flat == list(range(len(params))):
-        return phi
-    key = ("perm", str(device), tuple(flat))
-    inv = prog.words_cache.get(key)
-    if inv is None:
-        pos = [0] * len(flat)
-        for i, k in enumerate(flat):
-            pos[k] = i
-        inv = torch.tensor(pos, dtype=torch.long, device=device)
-        prog.words_cache[key] = inv
-    return phi.index_select(0, inv)
+    if flat != list(range(len(params))):
+        key = ("perm", str(device), tuple(flat))
+        inv = prog.words_cache.get(key)
+        if inv is None:
+            pos = [0] * len(flat)
+            for i, k in enumerate(flat):
+                pos[k] = i
+            inv = torch.tensor(pos, dtype=torch.long, device=device)
+            prog.words_cache[key] = inv
+        phi = phi.index_select(0, inv)
+    if any(sl.sign != 1.0 for sl in params):
+        key = ("signs", str(device))
+        sg = prog.words_cache.get(key)
+        if sg is None:
+            sg = torch.tensor([sl.sign for sl in params], dtype=torch.float32, device=device)
+            prog.words_cache[key] = sg
+        phi = phi * sg
+    return phi
 
 
 def coefficient_layout(prog: Program, B: int):
@@ -575,18 +599,19 @@ def specialised_kernels(prog: Program, pl: PL.Plan, ws: WordSet, B: int, gen_fir
 
 def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
             cm: PL.CostModel = None, want_state: bool = True, checkpoints=None, events=None,
-            specialise: bool = None):
+            specialise: bool = None, fold: bool = None):
     """Runs every pass.  Returns (state or None, probs or None).
 
     ``events``: optional list; a (start, end) pair of HIP events recorded on the
     launch stream around every pass kernel is appended to it.  ``specialise``:
     run compiled-per-program kernels (None: config.SPECIALISE) or the library's
     interpreting kernels (False; no compilation on first use of a program).
-    Compiled kernels fold trailing CNOTs into the readout of a measured
-    program (readout_fold): the state they leave is the one before those CNOTs."""
+    ``fold``: fold trailing CNOTs into the readout of a measured program
+    (readout_fold; compiled kernels only), so that the state left behind is
+    the one before those CNOTs; None folds when the state is not wanted."""
     n, B = prog.n, inp.B
     jit_on = use_jit(n, R, TB) if specialise is None else specialise
-    fold = folds(prog, jit_on)
+    fold = folds(prog, jit_on) and (not want_state if fold is None else bool(fold))
     pl = get_plan(prog, R, TB, cm, fold)
     ws = pass_words(prog, pl, B, inp.device)
     words = ws.passes

@@ -38,6 +38,14 @@ SIGNATURES = {
     "qs_measure_prob_backward": (I32, [P, P, P, I64, I64, I32, P]),
     "qs_pauli_im_work_bytes": (I64, [I64, I32, I32]),
     "qs_pauli_im": (I32, [P, P, P, I64, I32, P, I32, I32, P, I64, P]),
+    "qs_pauli_im_controlled": (I32, [P, P, P, I64, I32, P, I32, I32, I64, P, I64, P]),
+    "qs_unitary_1q": (I32, [P, P, I64, I32, I32, P, I64, I64, I32, P]),
+    "qs_rot_unitaries": (I32, [P, P, I64, I32, P]),
+    "qs_mcx": (I32, [P, P, I64, I32, I32, I64, P]),
+    "qs_phase_flip": (I32, [P, P, I64, I32, I64, P]),
+    "qs_swap": (I32, [P, P, I64, I32, I32, I32, I64, P]),
+    "qs_reset": (I32, [P, P, I64, I32, I32, P]),
+    "qs_reset_backward": (I32, [P, P, P, I64, I32, I32, P]),
     "qs_fused_supported": (I32, [I32, I32]),
     "qs_layout_supported": (I32, [I32, I32]),
     "qs_product_factors": (I32, [P, P, I64, I32, P, I32, I32, I64, P]),

@@ -183,3 +183,173 @@ class MeasureProbFn(torch.autograd.Function):
             "qs_measure_prob_backward",
         )
         return lam, None
+
+
+# ----------------------------------------------------------------------------
+# remaining operators (qs_gates.hip).  A control mask is a set of bit
+# positions that must all be 1 for the gate to act (0 = uncontrolled).
+# ----------------------------------------------------------------------------
+
+def rot_unitaries(cs: torch.Tensor, kind: int) -> torch.Tensor:
+    """(..., 8) float32 column-convention R_kind from (..., 2) (cos, sin) of phi/2."""
+    u = torch.empty(cs.shape[:-1] + (8,), dtype=torch.float32, device=cs.device)
+    count = cs.numel() // 2
+    if count:
+        L.check(L.load().qs_rot_unitaries(L.ptr(u), L.ptr(cs), count, kind, L.stream_of(cs)), "qs_rot_unitaries")
+    return u
+
+
+def _unitary(out, src, u, n: int, bit: int, cmask: int, dagger: int):
+    B = src.shape[0]
+    per_sample = u.dim() == 2
+    if per_sample and u.shape[0] != B:
+        raise RuntimeError(f"per-sample unitaries: {u.shape[0]} for a batch of {B}")
+    L.check(
+        L.load().qs_unitary_1q(L.ptr(out), L.ptr(src), B, n, bit, L.ptr(u), 1 if per_sample else 0, cmask, dagger,
+                               L.stream_of(src)),
+        "qs_unitary_1q",
+    )
+    return out
+
+
+class UnitaryFn(torch.autograd.Function):
+    """psi' = U psi on one bit position where the control mask holds; U is a
+    constant (8,) or per-sample (B, 8) float32 tensor (no gradient)."""
+
+    @staticmethod
+    def forward(ctx, state, u, n: int, bit: int, cmask: int):
+        state = _c64(state)
+        u = u.detach().to(device=state.device, dtype=torch.float32).contiguous()
+        ctx.save_for_backward(u)
+        ctx.meta = (n, bit, cmask)
+        return _unitary(torch.empty_like(state), state, u, n, bit, cmask, 0)
+
+    @staticmethod
+    def backward(ctx, g):
+        (u,) = ctx.saved_tensors
+        n, bit, cmask = ctx.meta
+        g = _c64(g)
+        return _unitary(torch.empty_like(g), g, u, n, bit, cmask, 1), None, None, None, None
+
+
+class ControlledRotFn(torch.autograd.Function):
+    """Controlled R_P(phi): psi' = R_P(phi) psi on one bit where the control
+    mask holds.  dL/dphi = 1/2 Im <g | P (x) |1..1><1..1| | psi'>."""
+
+    @staticmethod
+    def forward(ctx, state, phi, n: int, bit: int, kind: int, cmask: int):
+        state = _c64(state)
+        B = state.shape[0]
+        if phi.dim() == 1 and phi.shape[0] != B:
+            raise RuntimeError(f"per-sample angle has {phi.shape[0]} entries, state batch is {B}")
+        u = rot_unitaries(half_angle_coeffs(phi), kind)
+        out = _unitary(torch.empty_like(state), state, u, n, bit, cmask, 0)
+        ctx.save_for_backward(out, u)
+        ctx.meta = (n, bit, kind, cmask, phi.dim() == 1)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        out, u = ctx.saved_tensors
+        n, bit, kind, cmask, per_sample = ctx.meta
+        g = _c64(g)
+        gin = gphi = None
+        if ctx.needs_input_grad[0]:
+            gin = _unitary(torch.empty_like(g), g, u, n, bit, cmask, 1)
+        if ctx.needs_input_grad[1]:
+            B = out.shape[0]
+            lib = L.load()
+            wb = lib.qs_pauli_im_work_bytes(B, n, 1)
+            work = torch.empty(max(wb, 4), dtype=torch.uint8, device=out.device)
+            im = torch.empty(B, 1, dtype=torch.float32, device=out.device)
+            L.check(
+                lib.qs_pauli_im_controlled(L.ptr(im), L.ptr(g), L.ptr(out), B, n, L.i32_array([bit]), 1, kind, cmask,
+                                           L.ptr(work), wb, L.stream_of(out)),
+                "qs_pauli_im_controlled",
+            )
+            im = im[:, 0] * 0.5
+            gphi = im if per_sample else im.sum()
+        return gin, gphi, None, None, None, None
+
+
+class McxFn(torch.autograd.Function):
+    """X on bit ``tbit`` where the control mask holds (an involution)."""
+
+    @staticmethod
+    def forward(ctx, state, n: int, tbit: int, cmask: int):
+        ctx.meta = (n, tbit, cmask)
+        return McxFn._run(_c64(state), n, tbit, cmask)
+
+    @staticmethod
+    def _run(x, n, tbit, cmask):
+        out = torch.empty_like(x)
+        L.check(L.load().qs_mcx(L.ptr(out), L.ptr(x), x.shape[0], n, tbit, cmask, L.stream_of(x)), "qs_mcx")
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        n, tbit, cmask = ctx.meta
+        return McxFn._run(_c64(g), n, tbit, cmask), None, None, None
+
+
+class PhaseFlipFn(torch.autograd.Function):
+    """Negates the amplitudes whose bits ``mask`` are all 1 (diagonal, real)."""
+
+    @staticmethod
+    def forward(ctx, state, n: int, mask: int):
+        ctx.meta = (n, mask)
+        return PhaseFlipFn._run(_c64(state), n, mask)
+
+    @staticmethod
+    def _run(x, n, mask):
+        out = torch.empty_like(x)
+        L.check(L.load().qs_phase_flip(L.ptr(out), L.ptr(x), x.shape[0], n, mask, L.stream_of(x)), "qs_phase_flip")
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        n, mask = ctx.meta
+        return PhaseFlipFn._run(_c64(g), n, mask), None, None
+
+
+class SwapFn(torch.autograd.Function):
+    """Exchanges bits ``abit`` and ``bbit`` where the control mask holds (an involution)."""
+
+    @staticmethod
+    def forward(ctx, state, n: int, abit: int, bbit: int, cmask: int):
+        ctx.meta = (n, abit, bbit, cmask)
+        return SwapFn._run(_c64(state), n, abit, bbit, cmask)
+
+    @staticmethod
+    def _run(x, n, abit, bbit, cmask):
+        out = torch.empty_like(x)
+        L.check(L.load().qs_swap(L.ptr(out), L.ptr(x), x.shape[0], n, abit, bbit, cmask, L.stream_of(x)), "qs_swap")
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        n, abit, bbit, cmask = ctx.meta
+        return SwapFn._run(_c64(g), n, abit, bbit, cmask), None, None, None, None
+
+
+class ResetFn(torch.autograd.Function):
+    """Reset of one bit position (qs_reset); not unitary, gradient by qs_reset_backward."""
+
+    @staticmethod
+    def forward(ctx, state, n: int, bit: int):
+        state = _c64(state)
+        out = torch.empty_like(state)
+        L.check(L.load().qs_reset(L.ptr(out), L.ptr(state), state.shape[0], n, bit, L.stream_of(state)), "qs_reset")
+        ctx.save_for_backward(state)
+        ctx.meta = (n, bit)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        (state,) = ctx.saved_tensors
+        n, bit = ctx.meta
+        g = _c64(g)
+        gin = torch.empty_like(g)
+        L.check(L.load().qs_reset_backward(L.ptr(gin), L.ptr(g), L.ptr(state), g.shape[0], n, bit, L.stream_of(g)),
+                "qs_reset_backward")
+        return gin, None, None

@@ -259,6 +259,7 @@ k_measure_backward(float *lam, const float *psi, const float *w, int64_t w_strid
 struct PauliArgs {
   int32_t bits[kMaxBits];
   int d;
+  int64_t cmask;  // only amplitudes with every bit of cmask set contribute (0 = all)
 };
 
 // Stage 1 of Im<lam|P_j|psi>: part[ch][j]
@@ -285,7 +286,7 @@ k_pauli_stage1(float *part, const float *lam, const float *psi, int c, PauliArgs
         float2 q = ld2(psi, i);
         pp = one ? make_float2(-q.x, -q.y) : q;
       }
-      acc += l.x * pp.y - l.y * pp.x;  // Im(conj(l) pp)
+      if ((i & a.cmask) == a.cmask) acc += l.x * pp.y - l.y * pp.x;  // Im(conj(l) pp)
     }
     float s = block_sum(acc, red);
     if (threadIdx.x == 0) part[(int64_t)blockIdx.x * a.d + j] = s;
@@ -441,19 +442,20 @@ int64_t qs_pauli_im_work_bytes(int64_t B, int n, int d) {
   return (B << (n - c)) * (int64_t)d * (int64_t)sizeof(float);
 }
 
-int qs_pauli_im(float *out, const float *lam, const float *psi, int64_t B, int n,
-                const int32_t *bits, int d, int pauli, void *work, int64_t work_bytes,
-                void *stream) {
+int qs_pauli_im_controlled(float *out, const float *lam, const float *psi, int64_t B, int n,
+                           const int32_t *bits, int d, int pauli, int64_t cmask, void *work,
+                           int64_t work_bytes, void *stream) {
   if (bad_shape(B, n) || !out || !lam || !psi || !bits || !work || d < 1 || d > kMaxBits ||
-      pauli < 0 || pauli > 2)
+      pauli < 0 || pauli > 2 || cmask < 0 || (n < 63 && (cmask >> n) != 0))
     return QS_EINVAL;
   if (work_bytes < qs_pauli_im_work_bytes(B, n, d)) return QS_EWORKSPACE;
   PauliArgs a;
   for (int j = 0; j < d; ++j) {
-    if (bits[j] < 0 || bits[j] >= n) return QS_EINVAL;
+    if (bits[j] < 0 || bits[j] >= n || ((cmask >> bits[j]) & 1)) return QS_EINVAL;
     a.bits[j] = bits[j];
   }
   a.d = d;
+  a.cmask = cmask;
   const int c = chunk_bits(n);
   const int64_t nch = B << (n - c);
   if (nch > 0x7fffffff) return QS_EUNSUPPORTED;
@@ -467,6 +469,12 @@ int qs_pauli_im(float *out, const float *lam, const float *psi, int64_t B, int n
   hipLaunchKernelGGL(k_pauli_stage2, dim3((unsigned)B, (unsigned)d), dim3(kThreads), 0, st, out,
                      (const float *)part, n, c, d);
   return launch_status();
+}
+
+int qs_pauli_im(float *out, const float *lam, const float *psi, int64_t B, int n,
+                const int32_t *bits, int d, int pauli, void *work, int64_t work_bytes,
+                void *stream) {
+  return qs_pauli_im_controlled(out, lam, psi, B, n, bits, d, pauli, 0, work, work_bytes, stream);
 }
 
 }  // extern "C"
