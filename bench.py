@@ -81,13 +81,21 @@ def profile_workload(mode, reps=2):
     print(f"{mode}: {reps} runs of {w.name}", flush=True)
 
 
+def executed_plan(prog, R, TB):
+    """The plan a forward without state output runs: the compiled kernels fold
+    trailing CNOTs into the readout (fused.readout_fold)."""
+    from qandle_amd._engine import fused
+
+    return fused.get_plan(prog, R, TB, None, fused.folds(prog, fused.use_jit(prog.n, R, TB)))
+
+
 def fused_bytes_per_launch(B, n, npass, stores_last):
     """Algorithmic HBM bytes of the executed schedule, averaged per pass launch.
 
     First pass: generates the embedding product state in registers and writes
     it (8 B/amplitude).  Middle passes: read + write (16 B).  Last pass: read
     (8 B), plus a write when the state is kept.  Measurement partials are
-    (K+1) floats per tile, negligible.
+    K + 1 (or, with a folded readout, n + 1) floats per tile, negligible.
     """
     amps = B * 2 ** n
     per_amp = 8 + 16 * max(npass - 2, 0) + (8 + (8 if stores_last else 0) if npass > 1 else 0)
@@ -153,7 +161,7 @@ def main():
     value = updates * args.steps / gpu_s
 
     # per-launch durations of the dominant kernel, on its own stream, over K steps
-    plan = fused.get_plan(prog, R, TB)
+    plan = executed_plan(prog, R, TB)
     npass = len(plan.passes)
     with torch.no_grad():
         inp = fused.gather_inputs(prog, circ.circuit, None, {"x": x})
@@ -232,7 +240,7 @@ def main():
         with torch.no_grad():
             t24, _ = time_steps(lambda: c24(x=x24), 5, 2)
         p24 = fused.lower_circuit(c24.circuit)
-        np24 = len(fused.get_plan(p24, *fused.choose_shape(24)).passes)
+        np24 = len(executed_plan(p24, *fused.choose_shape(24)).passes)
         _, b24 = fused_bytes_per_launch(4, 24, np24, stores_last=False)
         extras["n24_updates_per_s"] = w24.amplitude_updates() * 5 / t24
         extras["n24_ms_per_step"] = t24 / 5 * 1e3

@@ -10,6 +10,7 @@ executed as a short sequence of ``qs_fused_pass`` launches.
 from __future__ import annotations
 
 import dataclasses
+import math
 import typing
 
 import numpy as np
@@ -50,9 +51,18 @@ class Program:
 
 def _lower_into(layers, n, mapping, prog_gates, slots, state):
     """Append the lowering of ``layers`` (qubits mapped through ``mapping``)."""
-    from .. import embeddings, measurements, operators, qcircuit
+    from .. import ansaetze, embeddings, measurements, operators, qcircuit
 
     for mod in layers:
+        if isinstance(mod, (ansaetze.StronglyEntanglingLayerBuilt, ansaetze.StronglyEntanglingLayerBudget)):
+            # plain gate sequences (src/qandle/ansaetze/stronglyentangling.py:109-110,
+            # stronglyentangling_budget.py:89-90)
+            if state.get("measured"):
+                return False
+            seq = mod.mods if isinstance(mod, ansaetze.StronglyEntanglingLayerBuilt) else mod.layers
+            if not _lower_into(list(seq), n, mapping, prog_gates, slots, state):
+                return False
+            continue
         if isinstance(mod, operators.BuiltParametrizedOperator):
             q = mapping(mod.qubit)
             slot = Slot("named", name=mod.name) if mod.named else Slot("param", module=mod)
@@ -70,6 +80,20 @@ def _lower_into(layers, n, mapping, prog_gates, slots, state):
             if g.a != g.b:
                 pa, pb = n - 1 - mapping(g.a), n - 1 - mapping(g.b)
                 prog_gates += [PL.Gate(PL.CNOT, pb, pa), PL.Gate(PL.CNOT, pa, pb), PL.Gate(PL.CNOT, pb, pa)]
+        elif isinstance(mod, operators.BuiltCZ) or (
+                isinstance(mod, operators.BuiltInvert) and isinstance(mod.target, operators.BuiltCZ)):
+            # CZ = H_t CNOT H_t with H = i RY(pi/2) RZ(pi) and the i^2 = -1 undone by
+            # RZ(2 pi) = -I: fixed-angle rotations that fuse into the target's
+            # per-bit unitaries, and a CNOT (its own inverse)
+            g = mod if isinstance(mod, operators.BuiltCZ) else mod.target
+            pt, pc = n - 1 - mapping(g.t), n - 1 - mapping(g.c)
+            for kind, v in ((PL.RZ, math.pi), (PL.RY, math.pi / 2), (PL.CNOT, None), (PL.RZ, math.pi),
+                            (PL.RY, math.pi / 2), (PL.RZ, 2 * math.pi)):
+                if kind == PL.CNOT:
+                    prog_gates.append(PL.Gate(PL.CNOT, pt, pc))
+                else:
+                    slots.append(Slot("value", value=v))
+                    prog_gates.append(PL.Gate(kind, pt, -1, len(slots) - 1))
         elif (isinstance(mod, operators.BuiltInvert) and isinstance(mod.target, operators.BuiltParametrizedOperator)
               and not mod.target.named):
             # R_P(phi)^-1 = R_P(-phi) (the reference inverts target.to_matrix(), src/qandle/operators.py:445-447)

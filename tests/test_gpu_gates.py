@@ -198,7 +198,7 @@ def test_circuit_with_remaining_operators(device):
                qandle.U(0, torch.from_numpy(M1)), qandle.Invert(qandle.RZ(5, theta=b, remapping=None)),
                qandle.CNOT(3, 2), qandle.Reset(4), qandle.MeasureProbability()]
     circ = qandle.Circuit(layers, num_qubits=n).to(device)
-    assert fused.lower_circuit(circ.circuit) is None  # CZ, Controlled, U, Reset: operator by operator
+    assert fused.lower_circuit(circ.circuit) is None  # Controlled, U, Reset: operator by operator
     with torch.no_grad():
         P = circ(x=x.to(device)).cpu().numpy()
     ops = [("embed", "y", list(range(n)), x.double().numpy())]

@@ -121,9 +121,9 @@ def test_reset_keeps_a_reset_qubit():
 
 
 def test_lowering_of_swap_and_inverses():
-    """SWAP lowers to three CNOTs, Invert(CNOT/SWAP) to themselves and
-    Invert(rotation) to the rotation with a negated slot; the rest stays
-    operator by operator."""
+    """SWAP lowers to three CNOTs, CZ to fixed-angle rotations around a CNOT,
+    Invert(CNOT/SWAP) to themselves and Invert(rotation) to the rotation with
+    a negated slot; the rest stays operator by operator."""
     import warnings
 
     import qandle_amd as qandle
@@ -140,7 +140,18 @@ def test_lowering_of_swap_and_inverses():
     assert [(g.kind, g.pos, g.ctrl) for g in prog.gates] == [
         (PL.CNOT, 1, 3), (PL.CNOT, 3, 1), (PL.CNOT, 1, 3), (PL.CNOT, 0, 2), (PL.RY, 2, -1)]
     assert prog.slots[prog.gates[-1].slot].sign == -1.0
-    for extra in (qandle.CZ(0, 1), qandle.U(0, torch.eye(2)), qandle.Reset(0), qandle.Controlled(0, qandle.RX(1))):
+    cz = qandle.Circuit([qandle.CZ(3, 1)], num_qubits=n)
+    p2 = fused.lower_circuit(cz.circuit)
+    # CZ(3, 1): target qubit 1 = position 2, control qubit 3 = position 0
+    assert [(g.kind, g.pos, g.ctrl) for g in p2.gates] == [
+        (PL.RZ, 2, -1), (PL.RY, 2, -1), (PL.CNOT, 2, 0), (PL.RZ, 2, -1), (PL.RY, 2, -1), (PL.RZ, 2, -1)]
+    # the decomposition is CZ itself (oracle, fp64)
+    ops = [(("rz", "ry", "cnot", "rz", "ry", "rz")[k], 1 if k != 2 else 3, v)
+           for k, v in enumerate((np.pi, np.pi / 2, None, np.pi, np.pi / 2, 2 * np.pi))]
+    ops = [("cnot", 3, 1) if o[0] == "cnot" else o for o in ops]
+    psi = fp64.simulate(n, [("ry", q, 0.3 + q) for q in range(n)])
+    np.testing.assert_allclose(fp64.simulate(n, ops, psi), fp64.apply_gate(psi, n, ("cz", 3, 1)), atol=1e-12)
+    for extra in (qandle.U(0, torch.eye(2)), qandle.Reset(0), qandle.Controlled(0, qandle.RX(1))):
         c2 = qandle.Circuit([qandle.RX(0), extra], num_qubits=n)
         assert fused.lower_circuit(c2.circuit) is None
 
