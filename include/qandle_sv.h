@@ -157,6 +157,27 @@ int qs_reset(float *out, const float *in, int64_t B, int n, int bit,
 int qs_reset_backward(float *gin, const float *g, const float *in, int64_t B,
                       int n, int bit, void *stream);
 
+/* AmplitudeEmbedding: out[b] (2^n complex64) = the first 2^n entries of
+ * (x[b, 0..d), pad, pad, ...) as real amplitudes, divided by
+ * max(||.||_2, 1e-12) when normalize != 0.  x is (B, d) float32 with row
+ * stride x_stride.  Replaces AmplitudeEmbeddingBuilt.forward
+ * (src/qandle/embeddings.py:50-61: F.pad with `pad_with`, F.normalize,
+ * torch.complex).  The backward writes gx (B, d) from g = dL/dRe + i dL/dIm;
+ * entries cropped away (d > 2^n) get 0. */
+int qs_amplitude_embed(float *out, const float *x, int64_t B, int d, int n,
+                       int64_t x_stride, float pad, int normalize,
+                       void *stream);
+int qs_amplitude_embed_backward(float *gx, const float *g, const float *x,
+                                int64_t B, int d, int n, int64_t x_stride,
+                                float pad, int normalize, void *stream);
+
+/* MeasureJointProbability: out[i] = |psi[i]|^2 over `count` amplitudes, and
+ * its gradient gpsi = 2 g psi.  Replaces MeasureAllAbsolute.forward
+ * (src/qandle/measurements.py:78-79). */
+int qs_abs2(float *out, const float *psi, int64_t count, void *stream);
+int qs_abs2_backward(float *gpsi, const float *g, const float *psi,
+                     int64_t count, void *stream);
+
 /* ---------------------------------------------------------------------------
  * Fused pass path.  A pass loads a tile of 2^K amplitudes (K = R + TB local
  * bit positions, lpos[] ascending with lpos[0] = 0) into LDS once, applies a

@@ -144,6 +144,29 @@ def apply_reset(psi: np.ndarray, n: int, q: int) -> np.ndarray:
     return out.reshape(B, 2 ** n)
 
 
+def amplitude_embed(X: np.ndarray, n: int, pad=None, normalize: bool = False) -> np.ndarray:
+    """AmplitudeEmbedding (``src/qandle/embeddings.py:50-61``): the features
+    padded with ``pad`` to 2^n entries (``F.pad``; a longer row is cropped),
+    divided by max(||row||, 1e-12) when ``normalize`` (``F.normalize``), as
+    real amplitudes."""
+    X = np.atleast_2d(np.asarray(X, dtype=np.float64))
+    N = 2 ** n
+    if X.shape[1] >= N:
+        V = X[:, :N].copy()
+    else:
+        if pad is None:
+            raise ValueError("features do not fill the register and no padding value is given")
+        V = np.concatenate([X, np.full((X.shape[0], N - X.shape[1]), float(pad))], axis=1)
+    if normalize:
+        V = V / np.maximum(np.linalg.norm(V, axis=1, keepdims=True), 1e-12)
+    return V.astype(np.complex128)
+
+
+def joint_prob(psi: np.ndarray) -> np.ndarray:
+    """MeasureJointProbability (``src/qandle/measurements.py:78-79``): |psi|^2."""
+    return np.abs(psi) ** 2
+
+
 def embed(X: np.ndarray, n: int, qubits, rot: str) -> np.ndarray:
     """(x)_j R(x_j)|0> over ``qubits``; unembedded qubits stay |0>."""
     X = np.asarray(X, dtype=np.float64)

@@ -14,6 +14,7 @@ from .measurements import *  # noqa: F401,F403
 from .qcircuit import *  # noqa: F401,F403
 from .splitter import split, SubcircuitContainer  # noqa: F401
 from .ansaetze import *  # noqa: F401,F403
-from . import operators, embeddings, measurements, qcircuit, splitter, ansaetze  # noqa: F401
+from .convolution import QConv  # noqa: F401
+from . import operators, embeddings, measurements, qcircuit, splitter, ansaetze, convolution  # noqa: F401
 
 __version__ = "0.1.0"

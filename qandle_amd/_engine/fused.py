@@ -39,8 +39,9 @@ class Program:
     n: int
     gates: typing.List[PL.Gate]
     slots: typing.List[Slot]
-    init: tuple                   # ("zero",) | ("state",) | ("embed", rot, bits, name)
+    init: tuple                   # ("input",) | ("embed", rot, bits, name) | ("amplitude", name, normalize, pad)
     measure: bool
+    post: str = None              # "abs2": MeasureJointProbability of the final state
     plans: dict = dataclasses.field(default_factory=dict)
     words_cache: dict = dataclasses.field(default_factory=dict)
 
@@ -110,6 +111,19 @@ def _lower_into(layers, n, mapping, prog_gates, slots, state):
             if mod.num_qubits != n or state.get("measured"):
                 return False
             state["measured"] = True
+        elif isinstance(mod, (measurements.MeasureAllAbsolute, measurements.MeasureState)):
+            # terminal readouts of the final state (src/qandle/measurements.py:73-91)
+            if state.get("measured") or state.get("post"):
+                return False
+            state["post"] = "abs2" if isinstance(mod, measurements.MeasureAllAbsolute) else "state"
+        elif isinstance(mod, embeddings.AmplitudeEmbeddingBuilt):
+            # the embedding replaces the state (src/qandle/embeddings.py:50-61): everything before it is dead;
+            # its amplitudes fill the whole register, so the qubit map must be the identity
+            if mod.num_qubits != n or any(mapping(q) != q for q in range(n)):
+                return False
+            prog_gates.clear()
+            slots.clear()
+            state["init"] = ("amplitude", mod.name, mod.normalize, mod.pad_with)
         elif isinstance(mod, qcircuit.Circuit):
             if mod.num_qubits != n or state.get("measured"):
                 return False
@@ -124,6 +138,9 @@ def _lower_into(layers, n, mapping, prog_gates, slots, state):
         else:
             return False
         if state.get("measured") and mod is not layers[-1] and not isinstance(mod, measurements.MeasureProbabilityBuilt):
+            return False
+        if state.get("post") and mod is not layers[-1] and not isinstance(
+                mod, (measurements.MeasureAllAbsolute, measurements.MeasureState)):
             return False
     return True
 
@@ -153,9 +170,12 @@ def lower_circuit(circuit) -> typing.Optional[Program]:
 
         if ok and state.get("measured"):
             ok = isinstance(layers[-1], measurements.MeasureProbabilityBuilt) or isinstance(layers[-1], qcircuit.Circuit)
+        if ok and state.get("post"):
+            ok = isinstance(layers[-1], (measurements.MeasureAllAbsolute, measurements.MeasureState))
     if not ok or n > 40 or n < 1:
         return None
-    prog = Program(n, gates, slots, state["init"], bool(state.get("measured")))
+    post = state.get("post")
+    prog = Program(n, gates, slots, state["init"], bool(state.get("measured")), "abs2" if post == "abs2" else None)
     circuit._program = prog
     return prog
 
@@ -319,6 +339,23 @@ def gather_inputs(prog: Program, circuit, state, kwargs) -> Inputs:
                 v = torch.tensor(v, dtype=torch.float32)
             named[name] = v
     embed_x = None
+    if prog.init[0] == "amplitude":
+        # the embedded amplitudes become the input state (autograd-tracked through the embedding)
+        from .plain import AmplitudeEmbedFn
+
+        _, name, normalize, pad = prog.init
+        xa = kwargs[name]
+        L.require_device(xa, f"input '{name}'")
+        N = 2 ** prog.n
+        if pad is None and xa.shape[-1] != N:
+            raise RuntimeError(f"AmplitudeEmbedding without pad_with needs {N} features, got {xa.shape[-1]}")
+        if xa.dim() not in (1, 2):
+            raise RuntimeError("AmplitudeEmbedding input must be (d,) or (B, d)")
+        batched_a = xa.dim() == 2
+        x2 = xa.reshape(-1, xa.shape[-1]).to(torch.float32)
+        state = AmplitudeEmbedFn.apply(x2, prog.n, 0.0 if pad is None else float(pad), bool(normalize))
+        device = state.device
+        B, batched = state.shape[0], batched_a
     if prog.init[0] == "embed":
         embed_x = kwargs[prog.init[3]]
         L.require_device(embed_x, f"input '{prog.init[3]}'")
@@ -644,7 +681,7 @@ def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
     utab, _ = compose_tables(ws, cs, generators=False)
     psi = torch.empty(B, 2 ** n, dtype=torch.complex64, device=inp.device)
     src = None
-    if prog.init[0] == "input" and inp.state is not None:
+    if prog.init[0] in ("input", "amplitude") and inp.state is not None:
         s = inp.state.to(torch.complex64).resolve_conj()
         if s.dim() == 1:
             s = s.unsqueeze(0)
@@ -709,7 +746,12 @@ def run(prog: Program, circuit, state, kwargs):
     if prog.measure:
         P = probs
         return measurements.reference_shape(P) if not inp.batched else _batched_probs(P)
-    return out_state if inp.batched else out_state.squeeze(0)
+    out = out_state if inp.batched else out_state.squeeze(0)
+    if prog.post == "abs2":
+        from .plain import Abs2Fn
+
+        out = Abs2Fn.apply(out)
+    return out
 
 
 def _batched_probs(P: torch.Tensor) -> torch.Tensor:

@@ -46,6 +46,10 @@ SIGNATURES = {
     "qs_swap": (I32, [P, P, I64, I32, I32, I32, I64, P]),
     "qs_reset": (I32, [P, P, I64, I32, I32, P]),
     "qs_reset_backward": (I32, [P, P, P, I64, I32, I32, P]),
+    "qs_amplitude_embed": (I32, [P, P, I64, I32, I32, I64, ctypes.c_float, I32, P]),
+    "qs_amplitude_embed_backward": (I32, [P, P, P, I64, I32, I32, I64, ctypes.c_float, I32, P]),
+    "qs_abs2": (I32, [P, P, I64, P]),
+    "qs_abs2_backward": (I32, [P, P, P, I64, P]),
     "qs_fused_supported": (I32, [I32, I32]),
     "qs_layout_supported": (I32, [I32, I32]),
     "qs_product_factors": (I32, [P, P, I64, I32, P, I32, I32, I64, P]),

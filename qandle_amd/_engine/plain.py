@@ -353,3 +353,51 @@ class ResetFn(torch.autograd.Function):
         L.check(L.load().qs_reset_backward(L.ptr(gin), L.ptr(g), L.ptr(state), g.shape[0], n, bit, L.stream_of(g)),
                 "qs_reset_backward")
         return gin, None, None
+
+
+class AmplitudeEmbedFn(torch.autograd.Function):
+    """(B, d) float32 features -> (B, 2^n) real amplitudes: padded with ``pad``
+    to 2^n (or cropped), L2-normalised when ``normalize``."""
+
+    @staticmethod
+    def forward(ctx, x, n: int, pad: float, normalize: bool):
+        x = x.detach().to(torch.float32).contiguous()
+        B, d = x.shape
+        out = torch.empty(B, 2 ** n, dtype=torch.complex64, device=x.device)
+        L.check(L.load().qs_amplitude_embed(L.ptr(out), L.ptr(x), B, d, n, d, float(pad), int(normalize),
+                                            L.stream_of(x)), "qs_amplitude_embed")
+        ctx.save_for_backward(x)
+        ctx.meta = (n, float(pad), int(normalize))
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        (x,) = ctx.saved_tensors
+        n, pad, normalize = ctx.meta
+        g = _c64(g)
+        gx = torch.empty_like(x)
+        B, d = x.shape
+        L.check(L.load().qs_amplitude_embed_backward(L.ptr(gx), L.ptr(g), L.ptr(x), B, d, n, d, pad, normalize,
+                                                     L.stream_of(x)), "qs_amplitude_embed_backward")
+        return gx, None, None, None
+
+
+class Abs2Fn(torch.autograd.Function):
+    """|psi|^2 elementwise (float32), gradient 2 g psi."""
+
+    @staticmethod
+    def forward(ctx, state):
+        state = _c64(state)
+        out = torch.empty(state.shape, dtype=torch.float32, device=state.device)
+        L.check(L.load().qs_abs2(L.ptr(out), L.ptr(state), state.numel(), L.stream_of(state)), "qs_abs2")
+        ctx.save_for_backward(state)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        (state,) = ctx.saved_tensors
+        g = g.to(torch.float32).contiguous()
+        gin = torch.empty_like(state)
+        L.check(L.load().qs_abs2_backward(L.ptr(gin), L.ptr(g), L.ptr(state), state.numel(), L.stream_of(state)),
+                "qs_abs2_backward")
+        return gin

@@ -143,6 +143,94 @@ k_reset_backward(float *gin, const float *g, const float *in, int64_t npairs, in
   }
 }
 
+// ---------------------------------------------------------------- embedding / readout
+// v = the first 2^n entries of (x[b, 0..d), pad, pad, ...); out[b] = v / max(||v||, 1e-12)
+// when normalize, else v (imaginary parts 0).  One block per sample.
+__device__ float gate_block_sum(float v, float *red) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < kGateThreads / 64; ++k) s += red[k];
+  return s;
+}
+
+__device__ __forceinline__ float amp_entry(const float *xb, int d, int64_t i, float pad) {
+  return i < d ? xb[i] : pad;
+}
+
+__global__ void __launch_bounds__(kGateThreads)
+k_amp_embed(float *out, const float *x, int d, int n, int64_t x_stride, float pad, int normalize) {
+  __shared__ float red[kGateThreads / 64];
+  const int64_t b = blockIdx.x, N = int64_t(1) << n;
+  const float *xb = x + b * x_stride;
+  float scale = 1.f;
+  if (normalize) {
+    float ss = 0.f;
+    for (int64_t i = threadIdx.x; i < N; i += kGateThreads) {
+      const float v = amp_entry(xb, d, i, pad);
+      ss += v * v;
+    }
+    scale = 1.f / fmaxf(sqrtf(gate_block_sum(ss, red)), 1e-12f);
+  }
+  for (int64_t i = threadIdx.x; i < N; i += kGateThreads)
+    st2(out, b * N + i, make_float2(amp_entry(xb, d, i, pad) * scale, 0.f));
+}
+
+// gx[b, i] (i < min(d, 2^n)) from g = dL/dRe + i dL/dIm of the embedded state:
+// without normalize gx = Re g; with it gx = Re g / r - v_i (v . Re g) / r^3, r = ||v||
+// (r <= 1e-12: the clamped norm is constant, gx = Re g / 1e-12)
+__global__ void __launch_bounds__(kGateThreads)
+k_amp_embed_backward(float *gx, const float *g, const float *x, int d, int n, int64_t x_stride, float pad,
+                     int normalize) {
+  __shared__ float red[kGateThreads / 64];
+  const int64_t b = blockIdx.x, N = int64_t(1) << n;
+  const int dm = d < N ? d : (int)N;
+  const float *xb = x + b * x_stride;
+  float r = 1.f, k = 0.f;
+  if (normalize) {
+    float ss = 0.f, vg = 0.f;
+    for (int64_t i = threadIdx.x; i < N; i += kGateThreads) {
+      const float v = amp_entry(xb, d, i, pad);
+      ss += v * v;
+      vg += v * ld2(g, b * N + i).x;
+    }
+    const float sumsq = gate_block_sum(ss, red);
+    const float dot = gate_block_sum(vg, red);
+    const float nrm = sqrtf(sumsq);
+    r = fmaxf(nrm, 1e-12f);
+    k = nrm > 1e-12f ? dot / (nrm * nrm * nrm) : 0.f;
+  }
+  for (int64_t i = threadIdx.x; i < dm; i += kGateThreads) {
+    const float gi = ld2(g, b * N + i).x;
+    gx[b * d + i] = normalize ? gi / r - xb[i] * k : gi;
+  }
+  for (int64_t i = dm + threadIdx.x; i < d; i += kGateThreads) gx[b * d + i] = 0.f;  // cropped entries
+}
+
+// out[i] = |psi[i]|^2 (MeasureJointProbability) and its gradient gpsi = 2 g psi
+__global__ void __launch_bounds__(kGateThreads) k_abs2(float *out, const float *psi, int64_t count) {
+  for (int64_t i = blockIdx.x * (int64_t)kGateThreads + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * kGateThreads) {
+    const float2 a = ld2(psi, i);
+    out[i] = a.x * a.x + a.y * a.y;
+  }
+}
+
+__global__ void __launch_bounds__(kGateThreads)
+k_abs2_backward(float *gpsi, const float *g, const float *psi, int64_t count) {
+  for (int64_t i = blockIdx.x * (int64_t)kGateThreads + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * kGateThreads) {
+    const float2 a = ld2(psi, i);
+    const float w = 2.f * g[i];
+    st2(gpsi, i, make_float2(w * a.x, w * a.y));
+  }
+}
+
 }  // namespace qs
 
 using namespace qs;
@@ -213,6 +301,37 @@ int qs_reset_backward(float *gin, const float *g, const float *in, int64_t B, in
   const int64_t npairs = (B << n) >> 1;
   hipLaunchKernelGGL(k_reset_backward, dim3(gate_grid(npairs)), dim3(kGateThreads), 0, (hipStream_t)stream, gin, g,
                      in, npairs, bit);
+  return launch_status();
+}
+
+int qs_amplitude_embed(float *out, const float *x, int64_t B, int d, int n, int64_t x_stride, float pad,
+                       int normalize, void *stream) {
+  if (gate_shape_bad(B, n) || !out || !x || d < 1 || x_stride < 0 || B > 0x7fffffff) return QS_EINVAL;
+  hipLaunchKernelGGL(k_amp_embed, dim3((unsigned)B), dim3(kGateThreads), 0, (hipStream_t)stream, out, x, d, n,
+                     x_stride, pad, normalize);
+  return launch_status();
+}
+
+int qs_amplitude_embed_backward(float *gx, const float *g, const float *x, int64_t B, int d, int n,
+                                int64_t x_stride, float pad, int normalize, void *stream) {
+  if (gate_shape_bad(B, n) || !gx || !g || !x || d < 1 || x_stride < 0 || B > 0x7fffffff) return QS_EINVAL;
+  hipLaunchKernelGGL(k_amp_embed_backward, dim3((unsigned)B), dim3(kGateThreads), 0, (hipStream_t)stream, gx, g,
+                     x, d, n, x_stride, pad, normalize);
+  return launch_status();
+}
+
+int qs_abs2(float *out, const float *psi, int64_t count, void *stream) {
+  if (count < 0 || (count > 0 && (!out || !psi))) return QS_EINVAL;
+  if (count == 0) return QS_OK;
+  hipLaunchKernelGGL(k_abs2, dim3(gate_grid(count)), dim3(kGateThreads), 0, (hipStream_t)stream, out, psi, count);
+  return launch_status();
+}
+
+int qs_abs2_backward(float *gpsi, const float *g, const float *psi, int64_t count, void *stream) {
+  if (count < 0 || (count > 0 && (!gpsi || !g || !psi))) return QS_EINVAL;
+  if (count == 0) return QS_OK;
+  hipLaunchKernelGGL(k_abs2_backward, dim3(gate_grid(count)), dim3(kGateThreads), 0, (hipStream_t)stream, gpsi, g,
+                     psi, count);
   return launch_status();
 }
 

@@ -16,7 +16,8 @@ from . import operators as op
 from . import utils
 from ._engine import plain
 
-__all__ = ["AngleEmbedding", "AngleEmbeddingBuilt", "InputOperator", "InputOperatorBuilt"]
+__all__ = ["AngleEmbedding", "AngleEmbeddingBuilt", "AmplitudeEmbedding", "AmplitudeEmbeddingBuilt",
+           "InputOperator", "InputOperatorBuilt"]
 
 
 class InputOperator(op.UnbuiltOperator):
@@ -29,6 +30,61 @@ class InputOperatorBuilt(op.BuiltOperator):
 
     def to_matrix(self, **kwargs):
         raise ValueError("Input operators do not have a matrix representation")
+
+
+class AmplitudeEmbeddingBuilt(InputOperatorBuilt):
+    """Ignores the incoming state and sets the features as the state
+    (``src/qandle/embeddings.py:29-65``): ``kwargs[name]`` (d,) or (B, d) is
+    padded with ``pad_with`` to 2^n entries (cropped when longer; without
+    ``pad_with`` it must already have 2^n), L2-normalised when ``normalize``,
+    and returned as real complex64 amplitudes (kernel ``qs_amplitude_embed``)."""
+
+    def __init__(self, name: str, num_qubits: int, normalize: bool, pad_with: typing.Union[float, None]):
+        super().__init__()
+        self.num_qubits = num_qubits
+        self.normalize = bool(normalize)
+        self.pad_with = pad_with
+        self.named = True
+        self.name = name
+
+    def embed(self, x: torch.Tensor) -> torch.Tensor:
+        """(B, 2^n) state of (B, d) features (autograd-tracked)."""
+        from ._engine.lib import require_device
+
+        require_device(x, f"input '{self.name}'")
+        N = 2 ** self.num_qubits
+        if self.pad_with is None and x.shape[-1] != N:
+            raise RuntimeError(f"AmplitudeEmbedding without pad_with needs {N} features, got {x.shape[-1]}")
+        x2 = x.reshape(-1, x.shape[-1]).to(torch.float32)
+        return plain.AmplitudeEmbedFn.apply(x2, self.num_qubits, 0.0 if self.pad_with is None else float(self.pad_with),
+                                            self.normalize)
+
+    def forward(self, _=None, **kwargs):
+        x = kwargs[self.name]
+        out = self.embed(x)
+        return out.reshape(x.shape[:-1] + (2 ** self.num_qubits,))
+
+    def __str__(self) -> str:
+        return "AmplitudeEmbedding"
+
+
+class AmplitudeEmbedding(InputOperator):
+    """Amplitude embedding before build (``src/qandle/embeddings.py:68-111``)."""
+
+    def __init__(self, name: str, qubits: list, normalize: bool = False, pad_with: typing.Union[float, None] = None):
+        self.name = name
+        self.named = True
+        self.qubits = qubits
+        self.pad_with = pad_with
+        self.normalize = normalize
+
+    def build(self, num_qubits: int, **kwargs) -> AmplitudeEmbeddingBuilt:
+        assert num_qubits == len(self.qubits), "Current Implementation requires all qubits to be used."
+        return AmplitudeEmbeddingBuilt(num_qubits=num_qubits, normalize=self.normalize, pad_with=self.pad_with,
+                                       name=self.name)
+
+    def __str__(self) -> str:
+        return "AmplitudeEmbedding"
 
 
 _ROT_KIND = {"RX": 0, "RY": 1, "RZ": 2}

@@ -13,7 +13,8 @@ import torch
 from . import operators as op
 from ._engine import plain
 
-__all__ = ["MeasureProbability", "MeasureProbabilityBuilt", "BuiltMeasurement", "UnbuiltMeasurement"]
+__all__ = ["MeasureProbability", "MeasureProbabilityBuilt", "BuiltMeasurement", "UnbuiltMeasurement",
+           "MeasureAllAbsolute", "MeasureJointProbability", "MeasureState"]
 
 
 def reference_shape(probs_bn: torch.Tensor) -> torch.Tensor:
@@ -35,6 +36,27 @@ class UnbuiltMeasurement(op.UnbuiltOperator):
 
     def build(self, num_qubits: int, **kwargs) -> BuiltMeasurement:
         return BUILT_CLASS_RELATION[self.__class__](num_qubits=num_qubits, **kwargs)
+
+
+class MeasureAllAbsolute(BuiltMeasurement):
+    """|amplitude|^2 of every basis state (``src/qandle/measurements.py:73-82``),
+    float32 with the state's shape (kernel ``qs_abs2``)."""
+
+    def forward(self, state: torch.Tensor) -> torch.Tensor:
+        from ._engine.lib import require_device
+
+        require_device(state, "state")
+        return plain.Abs2Fn.apply(state)
+
+
+MeasureJointProbability = MeasureAllAbsolute  # alias, as in the reference
+
+
+class MeasureState(BuiltMeasurement):
+    """The state itself (``src/qandle/measurements.py:85-91``): a no-op."""
+
+    def forward(self, state: torch.Tensor) -> torch.Tensor:
+        return state
 
 
 class MeasureProbability(UnbuiltMeasurement):

@@ -174,3 +174,23 @@ def test_modules_move_between_devices_and_keep_parameters():
     assert circ.circuit.layers[0]._u.dtype == torch.float32
     for m in circ.circuit.layers:
         assert isinstance(str(m), str)
+
+
+def test_amplitude_embedding_and_joint_probability_pins():
+    """AmplitudeEmbedding known answers (3-4-5 row, padding, cropping, unit
+    norm) and MeasureJointProbability summing to the squared norm."""
+    out = fp64.amplitude_embed(np.array([[3.0, 4.0]]), 2, pad=0, normalize=True)
+    np.testing.assert_allclose(out, [[0.6, 0.8, 0.0, 0.0]], atol=1e-15)
+    out = fp64.amplitude_embed(np.array([[1.0, 2.0, 3.0]]), 2, pad=7.0, normalize=False)
+    np.testing.assert_array_equal(out.real, [[1.0, 2.0, 3.0, 7.0]])
+    out = fp64.amplitude_embed(np.arange(10.0)[None, :], 3, pad=0, normalize=False)
+    np.testing.assert_array_equal(out.real, [np.arange(8.0)])
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal((5, 13))
+    v = fp64.amplitude_embed(X, 4, pad=0.5, normalize=True)
+    np.testing.assert_allclose(np.linalg.norm(v, axis=1), 1.0, atol=1e-14)
+    np.testing.assert_allclose(fp64.amplitude_embed(np.zeros((1, 4)), 2, normalize=True), 0.0)
+    with pytest.raises(ValueError):
+        fp64.amplitude_embed(X, 4)
+    psi = _state(4, seed=3) * 1.7
+    np.testing.assert_allclose(fp64.joint_prob(psi).sum(axis=1), 1.7 ** 2, atol=1e-12)
