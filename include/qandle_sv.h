@@ -221,6 +221,16 @@ int qs_compose_tables(float *utab, float *qtab, const int32_t *recipes,
                       const int32_t *offsets, int nrec, const float *cs,
                       void *stream);
 
+/* Matrix-core operands of composed unitaries (utab, 8 floats each): entry e
+ * (16 floats, ltab[16 e + 4 r + k]) is what lane row r = lane & 3 of a 4-lane
+ * block supplies to v_mfma_f32_4x4x1f32 for the real 4x4 form of U = utab
+ * entry idx[e] (e when idx = NULL; idx is device int32), or of U^dagger when
+ * dagger != 0.  The run-time specialised pass kernels copy their pass's
+ * entries into LDS and read each unitary's operands with one LDS load.  Part
+ * of the same per-call table build (src/qandle/operators.py:265-275). */
+int qs_lane_coeffs(float *ltab, const float *utab, const int32_t *idx,
+                   int64_t count, int dagger, void *stream);
+
 /* One pass.  src = NULL generates the product state of efac in registers;
  * otherwise tiles are read from src (src may equal state).  cs is the float2
  * (cos, sin) table of the phase entries and utab the fused unitaries that the

@@ -694,6 +694,14 @@ def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
         mpart = torch.empty(max(wb // 4, 1), dtype=torch.float32, device=inp.device)
     npass = len(words)
     kern = specialised_kernels(prog, pl, ws, B, src is None, fold) if jit_on else None
+    ltab = None
+    if kern is not None and kern.lanes:
+        # operands of the matrix-core unitaries: every table entry, or (lanes == "lds") each pass's block
+        idx = kern.lane_index_on(inp.device)
+        count = ws.nu if idx is None else idx.numel()
+        ltab = torch.empty(max(count, 1) * 16, dtype=torch.float32, device=inp.device)
+        L.check(lib.qs_lane_coeffs(L.ptr(ltab), L.ptr(utab), None if idx is None else L.ptr(idx), count, 0, st),
+                "qs_lane_coeffs")
     ntiles = tiles_of(B, n, R + TB)
     for i, (w, nsub, lpos) in enumerate(words):
         last = i == npass - 1
@@ -704,7 +712,8 @@ def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
         if kern is not None:
             kern.launch(i, L.ptr(psi), None if src is None else L.ptr(src), L.ptr(utab), L.ptr(cs),
                         None if efac is None else L.ptr(efac),
-                        L.ptr(mpart) if (last and mpart is not None) else None, B, ntiles, store, st)
+                        L.ptr(mpart) if (last and mpart is not None) else None, B, ntiles, store, st,
+                        None if ltab is None else L.ptr(ltab))
         else:
             L.check(
                 lib.qs_fused_pass(

@@ -42,6 +42,7 @@ typedef long long i64;
 typedef float qs_f4 __attribute__((ext_vector_type(4)));
 struct JArgs {
   float *state; const float *src; const int *utab; const int *cs; const float *efac; float *mpart;
+  const float *ltab;
   i64 B; i64 ntiles; int store; int pad;
 };
 __device__ __forceinline__ int swz(int l) { int f = (l >> 5) ^ (l >> 10) ^ (l >> 15); return l ^ (f & 31); }
@@ -161,7 +162,7 @@ __device__ __forceinline__ void valu_pair(float2 &x, float2 &y, const int *u) {
 }
 """
 
-ARGS_FMT = "<QQQQQQqqii"   # struct JArgs
+ARGS_FMT = "<QQQQQQQqqii"  # struct JArgs
 ADJ_FMT = "<QQQQQQQqqq"    # struct JAdj
 OPTIONS = ["--offload-arch=gfx950", "-O3", "-std=c++17",
            "-mllvm", "-simplifycfg-sink-common=false",
@@ -172,8 +173,21 @@ def _swz(l: int) -> int:
     return PL.swz(l)
 
 
+def lane_entries(words, R: int, TB: int, multi: bool):
+    """utab indices of the uniform (matrix-core) unitaries of a pass, distinct,
+    in program order: the pass's block of the lane-coefficient table."""
+    out = []
+    for sub in PL.decode_pass(words, R, TB):
+        for bits, _c in sub.segments:
+            for _r, ui, ps in bits:
+                if not ps and ui not in out:
+                    out.append(int(ui))
+    return out
+
+
 def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, measure: bool,
-                prefetch: bool = False, waves: int = None, readout=None, valu_pairs: int = 0) -> str:
+                prefetch: bool = False, waves: int = None, readout=None, valu_pairs: int = 0,
+                lanes=False, split_pairs: bool = False, lane_off: int = 0) -> str:
     """HIP source of one specialised forward pass kernel (extern "C" ``name``).
 
     ``prefetch`` keeps the next tile's HBM loads in flight in registers while
@@ -184,7 +198,15 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
     measures the bits themselves (K + 1 partials per slot by local bit,
     qs_fused_measure_finish).  ``valu_pairs``: of the amplitude pairs a
     thread updates per uniform fused unitary, that many run on the vector ALU
-    and the rest on the matrix cores (the two units then share the work)."""
+    and the rest on the matrix cores (the two units then share the work).
+    ``lanes``: the matrix-core operands come from the lane-coefficient table
+    (``qs_lane_coeffs``) instead of scalar reads and per-lane selects: True
+    loads each unitary's operands from it at the start of its sub-pass,
+    "lds" copies the pass's block (entries ``lane_off`` onward, in
+    ``lane_entries`` order) into LDS with the tile and reads them from there.
+    ``split_pairs``: tile loads and stores move each amplitude pair as two
+    8-byte LDS accesses (the swizzle may swap a pair's halves) instead of one
+    16-byte access with selects."""
     K = R + TB
     T, NR = 1 << TB, 1 << R
     multi = n < K
@@ -200,6 +222,12 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
     e(f"  __shared__ __align__(16) float2 tile[{1 << K}];")
     if measure and TB > 6:
         e(f"  __shared__ float red[{(T // 64) * ((K + 1) if readout is None else (n + 1))}];")
+    lane_k = None
+    if lanes == "lds":
+        ents = lane_entries(words, R, TB, multi)
+        lane_k = {ui: k for k, ui in enumerate(ents)}
+        if ents:
+            e(f"  __shared__ __align__(16) float4 ctab[{4 * len(ents)}];")
     e("  const int tid = threadIdx.x;")
     e(f"  const i64 total = A.B << {n};")
     e("  i64 g_tid = 0;")
@@ -235,11 +263,20 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
     if not gen and not pf:
         e("    fetch(base);")
     e("    const int ts = pinv(tid);")
+    if lane_k:
+        nl = 4 * len(lane_k)
+        e(f"    for (int c = ts; c < {nl}; c += {T}) ctab[c] = reinterpret_cast<const float4 *>(A.ltab)[{4 * lane_off} + c];")
+        if gen:
+            e("    __syncthreads();")
     if not gen:
         for it in range(NR // 2):
-            e(f"    {{ const int sl = swz(2 * ({it * T} + ts)); const float4 val = pre[{it}]; const bool sw = sl & 1;")
-            e("      *reinterpret_cast<float4 *>(&tile[sl & ~1]) = make_float4(sw ? val.z : val.x, sw ? val.w : val.y,"
-              " sw ? val.x : val.z, sw ? val.y : val.w); }")
+            if split_pairs:
+                e(f"    {{ const int sl = swz(2 * ({it * T} + ts)); const float4 val = pre[{it}];")
+                e("      tile[sl] = make_float2(val.x, val.y); tile[sl ^ 1] = make_float2(val.z, val.w); }")
+            else:
+                e(f"    {{ const int sl = swz(2 * ({it * T} + ts)); const float4 val = pre[{it}]; const bool sw = sl & 1;")
+                e("      *reinterpret_cast<float4 *>(&tile[sl & ~1]) = make_float4(sw ? val.z : val.x, sw ? val.w : val.y,"
+                  " sw ? val.x : val.z, sw ? val.y : val.w); }")
         e("    __syncthreads();")
         if pf:
             e("    { const i64 tn = t + gridDim.x; if (tn < A.ntiles) fetch(tile_base(tn)); }")
@@ -247,14 +284,17 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
     e(f"    float2 v[{NR}];")
     for s, sub in enumerate(subs):
         _sub_pass(e, sub, s, len(subs), n, R, TB, multi, gen and s == 0, measure and s == len(subs) - 1, readout,
-                  valu_pairs)
+                  valu_pairs, lanes=lane_k if lanes == "lds" else lanes)
     e("    if (A.store) {")
     e("      const int tz = pinv(tid);")
     for it in range(NR // 2):
         e(f"      {{ const i64 gi = base | g_tid | {goff[it]}LL; const int sl = swz(2 * ({it * T} + tz));")
-        e("        const float4 raw = *reinterpret_cast<const float4 *>(&tile[sl & ~1]); const bool sw = sl & 1;")
-        e("        const float4 val = make_float4(sw ? raw.z : raw.x, sw ? raw.w : raw.y, sw ? raw.x : raw.z,"
-          " sw ? raw.y : raw.w);")
+        if split_pairs:
+            e("        const float2 lo = tile[sl], hi = tile[sl ^ 1]; const float4 val = make_float4(lo.x, lo.y, hi.x, hi.y);")
+        else:
+            e("        const float4 raw = *reinterpret_cast<const float4 *>(&tile[sl & ~1]); const bool sw = sl & 1;")
+            e("        const float4 val = make_float4(sw ? raw.z : raw.x, sw ? raw.w : raw.y, sw ? raw.x : raw.z,"
+              " sw ? raw.y : raw.w);")
         if multi:
             e("        if (gi < total) reinterpret_cast<float4 *>(A.state)[gi >> 1] = val; }")
         else:
@@ -266,7 +306,118 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
     return "\n".join(o)
 
 
-def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu_pairs=0):
+def pass_source_pipelined(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, measure: bool,
+                          waves: int = None, readout=None, loaders: int = 8) -> str:
+    """HIP source of one specialised forward pass kernel with a load/compute
+    pipeline: a persistent workgroup holds two LDS tiles; 2^TB compute
+    threads run the sub-passes on one while 2^loaders loader threads store the
+    previous tile's result from, and load the next tile into, the other.  The
+    HBM traffic of a tile then overlaps the arithmetic of its neighbour inside
+    one workgroup.  Every barrier sits outside the role branches, so both
+    roles pass the same barriers in the same order.  Tiles of whole samples
+    (n < R + TB) are not pipelined."""
+    K = R + TB
+    T, NR = 1 << TB, 1 << R
+    TL = 1 << loaders
+    if n < K or loaders > K - 1:
+        raise ValueError("pipelined passes need tiles within one sample")
+    NL = (1 << (K - 1)) // TL  # float4 slots per loader thread
+    subs = PL.decode_pass(words, R, TB)
+    lpos = [int(x) for x in lpos]
+    npos = [p for p in range(n) if p not in lpos]
+    nn = len(npos)
+    o = []
+    e = o.append
+    wattr = f" __attribute__((amdgpu_waves_per_eu({waves})))" if waves else ""
+    e(f'extern "C" __global__ void __launch_bounds__({T + TL}){wattr}')
+    e(f"{name}(JArgs A) {{")
+    e(f"  __shared__ __align__(16) float2 buf[{2 << K}];")
+    if measure and TB > 6:
+        e(f"  __shared__ float red[{(T // 64) * ((K + 1) if readout is None else (n + 1))}];")
+    e(f"  const bool cmp = threadIdx.x < {T};")
+    e("  const int tid = cmp ? (int)threadIdx.x : 0;")
+    e(f"  const int lt = cmp ? 0 : (int)threadIdx.x - {T};")
+    e(f"  const i64 total = A.B << {n};")
+    e("  i64 g_ld = 0;")
+    for i in range(loaders):
+        e(f"  g_ld |= (i64)((lt >> {i}) & 1) << {lpos[i + 1]};")
+    loff = [sum(1 << lpos[loaders + 1 + i] for i in range(K - 1 - loaders) if (j >> i) & 1) for j in range(NL)]
+    e("  auto tile_base = [&](i64 t) -> i64 {")
+    e(f"    i64 b = (t >> {nn}) << {n};")
+    for i, p in enumerate(npos):
+        e(f"    b |= ((t >> {i}) & 1) << {p};")
+    e("    return b;")
+    e("  };")
+    e(f"  float4 ld[{NL}];")
+    e("  auto fetch = [&](i64 b) {")
+    for j in range(NL):
+        e(f"    ld[{j}] = reinterpret_cast<const float4 *>(A.src)[(b | g_ld | {loff[j]}LL) >> 1];")
+    e("  };")
+    e("  auto to_lds = [&](int bo) {")
+    e("    const int lz = pinv(lt);")
+    for j in range(NL):
+        e(f"    {{ const int sl = swz(2 * ({j * TL} + lz)); const float4 val = ld[{j}]; const bool sw = sl & 1;")
+        e("      *reinterpret_cast<float4 *>(&buf[bo + (sl & ~1)]) = make_float4(sw ? val.z : val.x, sw ? val.w : val.y,"
+          " sw ? val.x : val.z, sw ? val.y : val.w); }")
+    e("  };")
+    e("  auto store_tile = [&](i64 b, int bo) {")
+    e("    const int lz = pinv(lt);")
+    for j in range(NL):
+        e(f"    {{ const int sl = swz(2 * ({j * TL} + lz));")
+        e("      const float4 raw = *reinterpret_cast<const float4 *>(&buf[bo + (sl & ~1)]); const bool sw = sl & 1;")
+        e(f"      reinterpret_cast<float4 *>(A.state)[(b | g_ld | {loff[j]}LL) >> 1] = make_float4(sw ? raw.z : raw.x,"
+          " sw ? raw.w : raw.y, sw ? raw.x : raw.z, sw ? raw.y : raw.w); }")
+    e("  };")
+    e("  const i64 G = gridDim.x;")
+    e("  i64 t = blockIdx.x;")
+    if not gen:
+        e("  if (!cmp && t < A.ntiles) { fetch(tile_base(t)); to_lds(0); }")
+    e("  __syncthreads();")
+    e("  int k = 0;")
+    e("  for (; t < A.ntiles; ++k, t += G) {")
+    e(f"    const int boff = (k & 1) << {K};")
+    e("    const i64 base = tile_base(t);")
+    e(f"    const i64 block_sample = base >> {n};")
+    e("    if (!cmp) {")
+    e(f"      if (k > 0 && A.store) store_tile(tile_base(t - G), boff ^ {1 << K});")
+    if not gen:
+        e("      if (t + G < A.ntiles) fetch(tile_base(t + G));")
+    e("    }")
+    e(f"    float2 v[{NR}];")
+    for si, sub in enumerate(subs):
+        pipe = {}
+        e("    if (cmp) {")
+        _sub_pass(e, sub, si, len(subs), n, R, TB, False, gen and si == 0, measure and si == len(subs) - 1, readout,
+                  0, pipe)
+        e("    }")
+        e("    __syncthreads();")
+        if "finish" in pipe:
+            e("    if (cmp) {")
+            for line in pipe["finish"]:
+                e(line)
+            e("    }")
+    if not gen:
+        e(f"    if (!cmp && t + G < A.ntiles) to_lds(boff ^ {1 << K});")
+    e("    __syncthreads();")
+    e("  }")
+    e(f"  if (!cmp && k > 0 && A.store) store_tile(tile_base(t - G), ((k - 1) & 1) << {K});")
+    e("}")
+    return "\n".join(o)
+
+
+def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu_pairs=0, pipe=None, lanes=False):
+    """One sub-pass.  ``pipe`` (pipelined kernels): a dict; the tile is
+    ``buf[boff + i]``, no closing barrier is emitted, and a measurement's
+    cross-wave finish is left in ``pipe["finish"]`` for after the caller's
+    barrier."""
+    if pipe is not None:
+        import re
+
+        inner = e
+        sub_tile = re.compile(r"tile\[([^\]]*)\]")
+
+        def e(line):  # noqa: F811
+            inner(sub_tile.sub(r"buf[boff + (\1)]", line))
     K = R + TB
     T, NR = 1 << TB, 1 << R
     e(f"    {{  // sub-pass {s}")
@@ -286,6 +437,25 @@ def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu
             if (j >> r) & 1:
                 a ^= sw[r]
         swj.append(a)
+    cq = {}
+    if isinstance(lanes, dict):
+        # the sub-pass's operands are read from LDS up front (they are small and ready before the tile)
+        for si_, (bits, _c) in enumerate(sub.segments):
+            for r, ui, ps in bits:
+                if not ps and ui in lanes:
+                    nm = f"cq{si_}_{r}"
+                    cq[(si_, r)] = nm
+                    e(f"      const float4 {nm} = ctab[{4 * lanes[ui]} + (tq & 3)];")
+    elif lanes:
+        e("      const float *lt = A.ltab + 4 * (tq & 3);  // tq is pinned per sub-pass: the loads stay here")
+        for si_, (bits, _c) in enumerate(sub.segments):
+            for r, ui, ps in bits:
+                if ps and multi:
+                    continue
+                idx = f"({ui} + csample)" if ps else f"{ui}"
+                nm = f"cq{si_}_{r}"
+                cq[(si_, r)] = nm
+                e(f"      const float4 {nm} = *reinterpret_cast<const float4 *>(lt + 16 * {idx});")
     if gen:
         rmask = 0
         for r in range(R):
@@ -310,17 +480,24 @@ def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu
         for j in range(NR):
             e(f"      v[{j}] = tile[lb ^ {swj[j]}];")
     phys = list(range(NR))
-    for bits, cnots in sub.segments:
+    for si_, (bits, cnots) in enumerate(sub.segments):
         for r, ui, ps in bits:
             off = f"8 * ({ui} + csample)" if ps else f"{8 * ui}"
             e("      {")
-            e("        const int *ut = pin(A.utab);")
-            if ps and multi:
+            if (si_, r) in cq and not valu_pairs:
+                nm = cq[(si_, r)]
+                e(f"        const float m[4] = {{{nm}.x, {nm}.y, {nm}.z, {nm}.w}};")
+                for j in range(NR):
+                    if not (j >> r) & 1:
+                        e(f"        mfma_pair(v[{phys[j]}], v[{phys[j | (1 << r)]}], m);")
+            elif ps and multi:
+                e("        const int *ut = pin(A.utab);")
                 e(f"        const int *u = ut + {off};")
                 for j in range(NR):
                     if not (j >> r) & 1:
                         e(f"        valu_pair(v[{phys[j]}], v[{phys[j | (1 << r)]}], u);")
             else:
+                e("        const int *ut = pin(A.utab);")
                 e(f"        float m[4]; coeffs(ut + {off}, m);")
                 k = 0
                 for j in range(NR):
@@ -362,8 +539,9 @@ def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu
     for j in range(NR):
         e(f"      tile[lb ^ {swj[j]}] = v[{phys[j]}];")
     if measure:  # after the tile writes: the amplitudes' registers are free for the partials
-        _measure(e, sub, phys, n, R, TB, multi, readout)
-    e("      __syncthreads();")
+        _measure(e, sub, phys, n, R, TB, multi, readout, pipe)
+    if pipe is None:
+        e("      __syncthreads();")
     e("    }")
 
 
@@ -374,7 +552,7 @@ def _parity(var: str, mask: int) -> str:
     return f"(__popcll({var} & {mask}LL) & 1)"
 
 
-def _measure(e, sub, phys, n, R, TB, multi, readout=None):
+def _measure(e, sub, phys, n, R, TB, multi, readout=None, pipe=None):
     """MeasureProbability partials of the tile: K + 1 per slot by local bit
     (layout of qs_fused_measure_finish) or, with ``readout`` masks, n + 1 per
     slot by bit position (layout of qs_readout_finish)."""
@@ -441,14 +619,19 @@ def _measure(e, sub, phys, n, R, TB, multi, readout=None):
         e("        if ((tid & 63) == 0) {")
         e(f"          for (int q = 0; q < {W}; ++q) red[(tid >> 6) * {W} + q] = c[q];")
         e("        }")
-        e("        __syncthreads();")
-        e(f"        if (tid < {nsamp * W}) {{")
-        e(f"          const int sm = tid / {W}, q = tid % {W};")
-        e("          float x = 0.f;")
-        e(f"          for (int k = 0; k < {wps}; ++k) x += red[(sm * {wps} + k) * {W} + q];")
-        e(f"          const int dst[{W}] = {{{', '.join(str(d) for d in dst)}}};")
-        e(f"          mp[sm * {W} + dst[q]] = x;")
-        e("        }")
+        fin = [f"        if (tid < {nsamp * W}) {{",
+               f"          const int sm = tid / {W}, q = tid % {W};",
+               "          float x = 0.f;",
+               f"          for (int k = 0; k < {wps}; ++k) x += red[(sm * {wps} + k) * {W} + q];",
+               f"          const int dst[{W}] = {{{', '.join(str(d) for d in dst)}}};",
+               f"          A.mpart[((t << {spt_log}) + sm) * {W} + dst[q]] = x;",
+               "        }"]
+        if pipe is not None:
+            pipe["finish"] = fin
+        else:
+            e("        __syncthreads();")
+            for line in fin:
+                e(line)
     e("      }")
 
 
@@ -786,26 +969,55 @@ class ProgramKernels:
     ``waves`` = 2 asks for registers that let two waves share a SIMD: the LDS
     of a 2^13-amplitude tile already limits a compute unit to two workgroups,
     and without the bound the compiler spends accumulator registers until a
-    measuring pass drops to one (351 vs 256 us for that pass at 20 qubits)."""
+    measuring pass drops to one (351 vs 256 us for that pass at 20 qubits).
+    ``lanes`` = "lds" and ``split_pairs`` (pass_source) are the defaults: 3.48
+    vs 3.67 ms per 20-qubit forward, 4.69 vs 4.89 at 24 qubits, results bit
+    for bit the same."""
 
     def __init__(self, passes, n: int, R: int, TB: int, gen_first: bool, measure: bool,
                  prefetch: bool = False, waves: int = 2, readout=None, valu_pairs: int = 0,
-                 grid: str = "full"):
+                 grid: str = "full", pipeline: int = 0, lanes="lds", split_pairs: bool = True):
         self.n, self.R, self.TB = n, R, TB
-        self.block = 1 << TB
-        self.full_grid = grid == "full"
+        self.pipelined = bool(pipeline) and n >= R + TB
+        self.block = (1 << TB) + ((1 << pipeline) if self.pipelined else 0)
+        self.full_grid = grid == "full" and not self.pipelined
         npass = len(passes)
         self.sources, self.names = [], []
+        self.lane_index = []  # utab index of every lane-table entry (lanes == "lds"), pass by pass
         for i, (w, _nsub, lpos) in enumerate(passes):
             words = w.cpu().numpy() if hasattr(w, "cpu") else w
             nm = f"{KERNEL}_{i}"
-            self.sources.append(pass_source(nm, words, n, R, TB, lpos, gen_first and i == 0,
-                                            measure and i == npass - 1, prefetch, waves, readout, valu_pairs))
+            lane_off = len(self.lane_index)
+            if lanes == "lds" and not self.pipelined:
+                self.lane_index += lane_entries(words, R, TB, n < R + TB)
+            if self.pipelined:
+                self.sources.append(pass_source_pipelined(nm, words, n, R, TB, lpos, gen_first and i == 0,
+                                                          measure and i == npass - 1, waves, readout, pipeline))
+            else:
+                self.sources.append(pass_source(nm, words, n, R, TB, lpos, gen_first and i == 0,
+                                                measure and i == npass - 1, prefetch, waves, readout, valu_pairs,
+                                                lanes, split_pairs, lane_off))
             self.names.append(nm)
         self.funcs = kernels_for(self.sources, self.names)
         self.grid_cap = [None] * len(self.funcs)
+        self.lanes = (lanes if lanes == "lds" else bool(lanes)) and not self.pipelined
+        self._lane_idx = {}
 
-    def launch(self, i: int, state, src, utab, cs, efac, mpart, B: int, ntiles: int, store: int, stream):
+    def lane_index_on(self, device):
+        """The lane-table index list as a device int32 tensor (None unless lanes == "lds")."""
+        if self.lanes != "lds" or not self.lane_index:
+            return None
+        key = str(device)
+        t = self._lane_idx.get(key)
+        if t is None:
+            import torch
+
+            t = torch.tensor(self.lane_index, dtype=torch.int32, device=device)
+            self._lane_idx[key] = t
+        return t
+
+    def launch(self, i: int, state, src, utab, cs, efac, mpart, B: int, ntiles: int, store: int, stream,
+               ltab=None):
         lib = L.load()
         if self.full_grid:
             grid = max(1, min(ntiles, 0x7fffffff))
@@ -814,7 +1026,7 @@ class ProgramKernels:
                 g = lib.qs_jit_resident_blocks(self.funcs[i], self.block)
                 self.grid_cap[i] = int(g) if g and g > 0 else 2048
             grid = max(1, min(ntiles, self.grid_cap[i]))
-        args = struct.pack(ARGS_FMT, state or 0, src or 0, utab or 0, cs or 0, efac or 0, mpart or 0,
+        args = struct.pack(ARGS_FMT, state or 0, src or 0, utab or 0, cs or 0, efac or 0, mpart or 0, ltab or 0,
                            B, ntiles, store, 0)
         buf = ctypes.create_string_buffer(args, len(args))
         L.check(lib.qs_jit_launch(self.funcs[i], grid, self.block, buf, len(args), stream), "qs_jit_launch")

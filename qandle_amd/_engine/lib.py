@@ -69,6 +69,7 @@ SIGNATURES = {
     "qs_fused_measure_work_bytes": (I64, [I64, I32, I32, I32]),
     "qs_fused_measure_finish": (I32, [P, P, I64, I32, I32, I32, P, P]),
     "qs_readout_work_bytes": (I64, [I64, I32, I32, I32]),
+    "qs_lane_coeffs": (I32, [P, P, P, I64, I32, P]),
     "qs_readout_finish": (I32, [P, P, I64, I32, I32, I32, P]),
     "qs_adjoint_work_bytes": (I64, [I64, I32, I32, I32, I32]),
     "qs_adjoint_pass": (I32, [P, P, I64, I32, I32, I32, P, P, I32, P, P, P, P, I32, P, P]),

@@ -551,6 +551,33 @@ __device__ __forceinline__ M2 recipe_rot(const int32_t *w, int k, int64_t b, con
   return rot_matrix(kind, v.x, v.y);
 }
 
+// Per-lane matrix-core operands of fused unitaries: for v_mfma_f32_4x4x1f32,
+// lane row r = lane & 3 of a 4-lane block supplies A[r][k] of the real 4x4 form
+// of U (or, with dagger, of U^dagger, its transpose).  Entry e (16 floats,
+// ltab[16 e + 4 r + k]) is made from utab entry idx[e] (e when idx is NULL).
+__global__ void k_lane_coeffs(float *ltab, const float *utab, const int32_t *idx, int64_t count, int dagger) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count * 4;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t u = e >> 2;
+    const int r = (int)(e & 3);
+    const float *m = utab + 8 * (idx ? (int64_t)idx[u] : u);
+    const float a = m[0], b = m[1], c = m[2], d = m[3], f = m[5], g = m[6], h = m[7], ee = m[4];
+    float4 o;
+    if (!dagger) {
+      if (r == 0) o = make_float4(a, -b, c, -d);
+      else if (r == 1) o = make_float4(b, a, d, c);
+      else if (r == 2) o = make_float4(ee, -f, g, -h);
+      else o = make_float4(f, ee, h, g);
+    } else {
+      if (r == 0) o = make_float4(a, b, ee, f);
+      else if (r == 1) o = make_float4(-b, a, -f, ee);
+      else if (r == 2) o = make_float4(c, d, g, h);
+      else o = make_float4(-d, c, -h, g);
+    }
+    reinterpret_cast<float4 *>(ltab)[e] = o;
+  }
+}
+
 __global__ void __launch_bounds__(64)
 k_compose(float *utab, float *qtab, const int32_t *words, const int32_t *off, int nrec,
           const float *cs) {
@@ -697,6 +724,17 @@ int qs_compose_tables(float *utab, float *qtab, const int32_t *recipes, const in
   const unsigned g = (unsigned)(nrec < 65536 ? nrec : 65536);
   hipLaunchKernelGGL(k_compose, dim3(g), dim3(64), 0, (hipStream_t)stream, utab, qtab, recipes, offsets,
                      nrec, cs);
+  return launch_status();
+}
+
+int qs_lane_coeffs(float *ltab, const float *utab, const int32_t *idx, int64_t count, int dagger,
+                   void *stream) {
+  if (count < 0 || (count > 0 && (!ltab || !utab))) return QS_EINVAL;
+  if (count == 0) return QS_OK;
+  int64_t blocks = (count * 4 + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_lane_coeffs, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, ltab, utab, idx,
+                     count, dagger);
   return launch_status();
 }
 

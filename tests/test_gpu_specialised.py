@@ -184,3 +184,35 @@ def test_specialised_folded_readout_and_gradients(device, specialise, n, B):
     assert np.abs(P1 - ref).max() < ATOL and np.abs(P1 - P0).max() < 1e-6
     assert np.abs(gx1 - adj[0]).max() < 1e-4 and np.abs(gt1 - ot).max() < 1e-4
     assert np.abs(gx1 - gx0).max() < 1e-5 and np.abs(gt1 - gt0).max() < 1e-5
+
+
+@pytest.mark.parametrize("opts", [{"pipeline": 8}, {"lanes": True}, {"lanes": False, "split_pairs": False},
+                                  {"grid": "resident", "prefetch": True}])
+def test_pass_kernel_variants_are_bit_identical(device, specialise, opts):
+    """The pass-kernel variants (pipelined load/compute roles, operand loads,
+    scalar operands with 16-byte tile moves, a resident grid with prefetch)
+    compute exactly what the default kernels do."""
+    from qandle_amd._engine import fused
+
+    n, B = 16, 3
+    w = workloads.rxryrz_ladder(n=n, B=B, depth=3, seed=17)
+    circ = qandle.Circuit(w.layers, num_qubits=n).to(device)
+    prog = fused.lower_circuit(circ.circuit)
+    config.SPECIALISE = True
+    old = config.SPECIALISE_OPTIONS
+    try:
+        with torch.no_grad():
+            inp = fused.gather_inputs(prog, circ.circuit, None, {"x": w.inputs["x"].to(device)})
+            cs = fused.build_coeffs(prog, inp)
+            efac = fused.product_factors(prog, inp)
+            R, TB = fused.choose_shape(n)
+            config.SPECIALISE_OPTIONS = {}
+            s0, p0 = fused.execute(prog, inp, cs, efac, R, TB, want_state=True)
+            s0, p0 = s0.clone(), p0.clone()
+            config.SPECIALISE_OPTIONS = opts
+            s1, p1 = fused.execute(prog, inp, cs, efac, R, TB, want_state=True)
+    finally:
+        config.SPECIALISE_OPTIONS = old
+    assert torch.equal(s0, s1) and torch.equal(p0, p1)
+    ref = fp64.measure_prob(fp64.simulate(n, w.oracle_ops()), n)
+    assert np.abs(p1.cpu().numpy() - ref).max() < ATOL

@@ -70,3 +70,28 @@ def test_generated_adjoint_kernels_compile(n, R, TB, named, tmp_path, monkeypatc
     assert "im_trace" in src and "A.gpart" in src
     code = jit._code_for(src)
     assert len(code) > 1000
+
+
+@pytest.mark.parametrize("opts", [{"lanes": "lds", "split_pairs": True}, {"lanes": True}, {"lanes": False,
+                                                                                         "split_pairs": False}])
+def test_pass_kernel_options_compile(opts, tmp_path, monkeypatch):
+    """The operand-table and tile-move variants of the pass kernels (the engine
+    default is lanes="lds" with split pairs) and the pipelined form compile."""
+    monkeypatch.setattr(jit, "CACHE_DIR", str(tmp_path))
+    n, R, TB = 14, 5, 8
+    prog = _program(n, True)
+    B = 3
+    pl = fused.get_plan(prog, R, TB, PL.CostModel(budget=48.0, min_low=2))
+    ws = fused.pass_words(prog, pl, B, "cpu")
+    last = len(ws.passes) - 1
+    off = 0
+    srcs = []
+    for i, (w, _nsub, lpos) in enumerate(ws.passes):
+        srcs.append(jit.pass_source(f"qs_pass_{i}", w.numpy(), n, R, TB, lpos, i == 0, i == last, waves=2,
+                                    lane_off=off, **opts))
+        off += len(jit.lane_entries(w.numpy(), R, TB, False))
+    if opts.get("lanes") == "lds":
+        assert "ctab[" in srcs[0]
+    srcs.append(jit.pass_source_pipelined("qs_pipe", ws.passes[1][0].numpy(), n, R, TB, ws.passes[1][2], False,
+                                          False, 2, None, 8))
+    assert len(jit._code_for("\n\n".join(srcs))) > 1000
