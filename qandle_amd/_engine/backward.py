@@ -110,14 +110,20 @@ class FusedCircuitFn(torch.autograd.Function):
         gpart = torch.empty(max(wb // 4, 1), dtype=torch.float32, device=psi.device)
         npass = len(pl.passes)
         kern = None
+        ltab = None
         if ctx.jit:
             kern = specialised_adjoint(prog, pl, ws, B, embed, seed is not None, ctx.fold)
             ntiles, NS = adjoint_geometry(B, n, R, TB)
+            idx = kern.lane_index_on(psi.device)
+            if idx is not None:  # U^dagger operands of the matrix-core unitaries, step by step
+                ltab = torch.empty(idx.numel() * 16, dtype=torch.float32, device=psi.device)
+                L.check(lib.qs_lane_coeffs(L.ptr(ltab), L.ptr(utab), L.ptr(idx), idx.numel(), 1, st),
+                        "qs_lane_coeffs")
         for k, (w, nsub, lpos) in enumerate(ws.passes):
             seed_ptr = None if (k > 0 or seed is None) else L.ptr(seed)
             if kern is not None:
                 kern.launch(k, L.ptr(psi), L.ptr(lam), L.ptr(utab), L.ptr(qtab), L.ptr(cs), L.ptr(gpart), seed_ptr,
-                            B, NS, ntiles, st)
+                            B, NS, ntiles, st, None if ltab is None else L.ptr(ltab))
             else:
                 L.check(
                     lib.qs_adjoint_pass(

@@ -95,3 +95,19 @@ def test_pass_kernel_options_compile(opts, tmp_path, monkeypatch):
     srcs.append(jit.pass_source_pipelined("qs_pipe", ws.passes[1][0].numpy(), n, R, TB, ws.passes[1][2], False,
                                           False, 2, None, 8))
     assert len(jit._code_for("\n\n".join(srcs))) > 1000
+
+
+def test_adjoint_lane_table_variant_compiles(tmp_path, monkeypatch):
+    monkeypatch.setattr(jit, "CACHE_DIR", str(tmp_path))
+    n, R, TB = 13, 4, 8
+    prog = _program(n, False)
+    B = 3
+    pl = fused.get_plan(prog, R, TB, PL.CostModel(budget=48.0, min_low=2))
+    ws, nmain = fused.adjoint_words(prog, pl, B, "cpu", True)
+    off, srcs = 0, []
+    for i, (w, _nsub, lpos) in enumerate(ws.passes):
+        srcs.append(jit.adjoint_source(f"qs_adj_{i}", w.numpy(), n, R, TB, lpos, i == 0, waves=2, lanes="lds",
+                                       split_pairs=True, lane_off=off))
+        off += len(jit.adjoint_lane_entries(w.numpy(), R, TB))
+    assert off > 0 and any("ctab[" in s_ for s_ in srcs)
+    assert len(jit._code_for("\n\n".join(srcs))) > 1000
