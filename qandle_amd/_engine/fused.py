@@ -124,6 +124,8 @@ def _lower_into(layers, n, mapping, prog_gates, slots, state):
             prog_gates.clear()
             slots.clear()
             state["init"] = ("amplitude", mod.name, mod.normalize, mod.pad_with)
+        elif isinstance(mod, measurements.MeasurementDummy):
+            pass  # a decomposed measurement leaves the state as it is
         elif isinstance(mod, qcircuit.Circuit):
             if mod.num_qubits != n or state.get("measured"):
                 return False
@@ -696,12 +698,14 @@ def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
     kern = specialised_kernels(prog, pl, ws, B, src is None, fold) if jit_on else None
     ltab = None
     if kern is not None and kern.lanes:
-        # operands of the matrix-core unitaries: every table entry, or (lanes == "lds") each pass's block
-        idx = kern.lane_index_on(inp.device)
-        count = ws.nu if idx is None else idx.numel()
-        ltab = torch.empty(max(count, 1) * 16, dtype=torch.float32, device=inp.device)
-        L.check(lib.qs_lane_coeffs(L.ptr(ltab), L.ptr(utab), None if idx is None else L.ptr(idx), count, 0, st),
-                "qs_lane_coeffs")
+        # operands of the matrix-core unitaries: every table entry (lanes True), or (lanes "lds") each
+        # pass's block of the entries it uses, packed pass after pass; none when no pass uses one
+        idx = kern.lane_index_on(inp.device) if kern.lanes == "lds" else None
+        if kern.lanes is True or idx is not None:
+            count = ws.nu if idx is None else idx.numel()
+            ltab = torch.empty(max(count, 1) * 16, dtype=torch.float32, device=inp.device)
+            L.check(lib.qs_lane_coeffs(L.ptr(ltab), L.ptr(utab), None if idx is None else L.ptr(idx), count, 0,
+                                       st), "qs_lane_coeffs")
     ntiles = tiles_of(B, n, R + TB)
     for i, (w, nsub, lpos) in enumerate(words):
         last = i == npass - 1

@@ -85,6 +85,24 @@ __device__ __forceinline__ void mfma_pair(float2 &x, float2 &y, const float (&m)
   x = make_float2(acc[0], acc[1]);
   y = make_float2(acc[2], acc[3]);
 }
+// two independent pairs with their MFMA chains interleaved: each dependent
+// accumulation then finds the other chain's MFMA between it and its producer
+// instead of wait states
+__device__ __forceinline__ void mfma_pair2(float2 &x0, float2 &y0, float2 &x1, float2 &y1, const float (&m)[4]) {
+  qs_f4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = {0.f, 0.f, 0.f, 0.f};
+  a0 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[0], x0.x, a0, 0, 0, 0);
+  a1 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[0], x1.x, a1, 0, 0, 0);
+  a0 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[1], x0.y, a0, 0, 0, 0);
+  a1 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[1], x1.y, a1, 0, 0, 0);
+  a0 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[2], y0.x, a0, 0, 0, 0);
+  a1 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[2], y1.x, a1, 0, 0, 0);
+  a0 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[3], y0.y, a0, 0, 0, 0);
+  a1 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[3], y1.y, a1, 0, 0, 0);
+  x0 = make_float2(a0[0], a0[1]);
+  y0 = make_float2(a0[2], a0[3]);
+  x1 = make_float2(a1[0], a1[1]);
+  y1 = make_float2(a1[2], a1[3]);
+}
 // coefficients of U^dagger (the transposed real 4x4 form): A_k = U[k][lane & 3]
 __device__ __forceinline__ void coeffs_dag(const int *up, float (&m)[4]) {
   cint *u = as_const(up);
@@ -188,7 +206,7 @@ def lane_entries(words, R: int, TB: int, multi: bool):
 
 def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, measure: bool,
                 prefetch: bool = False, waves: int = None, readout=None, valu_pairs: int = 0,
-                lanes=False, split_pairs: bool = False, lane_off: int = 0) -> str:
+                lanes=False, split_pairs: bool = False, lane_off: int = 0, interleave: bool = False) -> str:
     """HIP source of one specialised forward pass kernel (extern "C" ``name``).
 
     ``prefetch`` keeps the next tile's HBM loads in flight in registers while
@@ -207,7 +225,10 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
     ``lane_entries`` order) into LDS with the tile and reads them from there.
     ``split_pairs``: tile loads and stores move each amplitude pair as two
     8-byte LDS accesses (the swizzle may swap a pair's halves) instead of one
-    16-byte access with selects."""
+    16-byte access with selects.  ``interleave``: the matrix-core updates of a
+    lane-table unitary run two amplitude pairs at a time with their MFMA
+    chains interleaved (mfma_pair2), which fills the dependent-accumulation
+    wait states; same arithmetic per pair, so results are bit for bit equal."""
     K = R + TB
     T, NR = 1 << TB, 1 << R
     multi = n < K
@@ -285,7 +306,7 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
     e(f"    float2 v[{NR}];")
     for s, sub in enumerate(subs):
         _sub_pass(e, sub, s, len(subs), n, R, TB, multi, gen and s == 0, measure and s == len(subs) - 1, readout,
-                  valu_pairs, lanes=lane_k if lanes == "lds" else lanes)
+                  valu_pairs, lanes=lane_k if lanes == "lds" else lanes, interleave=interleave)
     e("    if (A.store) {")
     e("      const int tz = pinv(tid);")
     for it in range(NR // 2):
@@ -406,7 +427,8 @@ def pass_source_pipelined(name: str, words, n: int, R: int, TB: int, lpos, gen: 
     return "\n".join(o)
 
 
-def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu_pairs=0, pipe=None, lanes=False):
+def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu_pairs=0, pipe=None, lanes=False,
+              interleave=False):
     """One sub-pass.  ``pipe`` (pipelined kernels): a dict; the tile is
     ``buf[boff + i]``, no closing barrier is emitted, and a measurement's
     cross-wave finish is left in ``pipe["finish"]`` for after the caller's
@@ -488,9 +510,16 @@ def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu
             if (si_, r) in cq and not valu_pairs:
                 nm = cq[(si_, r)]
                 e(f"        const float m[4] = {{{nm}.x, {nm}.y, {nm}.z, {nm}.w}};")
-                for j in range(NR):
-                    if not (j >> r) & 1:
-                        e(f"        mfma_pair(v[{phys[j]}], v[{phys[j | (1 << r)]}], m);")
+                prs = [(phys[j], phys[j | (1 << r)]) for j in range(NR) if not (j >> r) & 1]
+                step = 2 if interleave else 1
+                for q in range(0, len(prs) - step + 1, step):
+                    if interleave:
+                        (a0, b0), (a1, b1) = prs[q], prs[q + 1]
+                        e(f"        mfma_pair2(v[{a0}], v[{b0}], v[{a1}], v[{b1}], m);")
+                    else:
+                        e(f"        mfma_pair(v[{prs[q][0]}], v[{prs[q][1]}], m);")
+                if interleave and len(prs) % 2:
+                    e(f"        mfma_pair(v[{prs[-1][0]}], v[{prs[-1][1]}], m);")
             elif ps and multi:
                 e("        const int *ut = pin(A.utab);")
                 e(f"        const int *u = ut + {off};")
@@ -650,7 +679,7 @@ def adjoint_lane_entries(words, R: int, TB: int):
 
 def adjoint_source(name: str, words, n: int, R: int, TB: int, lpos, seed: bool,
                    prefetch: bool = False, waves: int = None, readout=None, lanes=False,
-                   split_pairs: bool = False, lane_off: int = 0) -> str:
+                   split_pairs: bool = False, lane_off: int = 0, interleave: bool = False) -> str:
     """HIP source of one specialised adjoint pass kernel (the adjoint tile of
     qs_adjoint_pass with the pass program compiled in).  ``seed``: the first
     step of the sweep, whose co-state is built from psi and w = dL/dP, with
@@ -761,7 +790,7 @@ def adjoint_source(name: str, words, n: int, R: int, TB: int, lpos, seed: bool,
     e(f"    const bool leader = (tid & {(1 << wlog) - 1}) == 0;")
     e(f"    float2 vp[{NR}], vl[{NR}];")
     for s, sub in enumerate(subs):
-        _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog, lane_k)
+        _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog, lane_k, interleave)
     e("    const int tz = pinv(tid);")
     for it in range(NR // 2):
         e(f"    {{ const i64 gi = base | g_tid | {goff[it]}LL; const int sl = swz(2 * ({it * T} + tz)); const bool sw = sl & 1;")
@@ -812,7 +841,7 @@ def _emit_put(e, expr: str, row: int, wlog: int, indent: str):
     e(f"{indent}  if (leader) pinf(A.gpart)[(i64){row} * A.NS + uslot] = g; }}")
 
 
-def _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog, lanes=None):
+def _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog, lanes=None, interleave=False):
     NR = 1 << R
     e(f"    {{  // adjoint sub-pass {s}")
     e("      const int tq = pinv(tid);")
@@ -909,7 +938,10 @@ def _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog, lanes=None):
                     for j in range(NR):
                         if not (j >> r) & 1:
                             a, b = phys[j], phys[j | (1 << r)]
-                            e(f"        mfma_pair(vp[{a}], vp[{b}], cf); mfma_pair(vl[{a}], vl[{b}], cf);")
+                            if interleave:
+                                e(f"        mfma_pair2(vp[{a}], vp[{b}], vl[{a}], vl[{b}], cf);")
+                            else:
+                                e(f"        mfma_pair(vp[{a}], vp[{b}], cf); mfma_pair(vl[{a}], vl[{b}], cf);")
                 else:
                     e(f"        float cf[4]; coeffs_dag(ut + {off}, cf);")
                     for j in range(NR):
@@ -1012,6 +1044,15 @@ def kernels_for(sources, names):
     return [hit[1][nm] for nm in names]
 
 
+def lane_mode(lanes, pipelined: bool):
+    """How a forward program's kernels take their matrix-core operands: "lds"
+    (a compact per-pass block, lane_index), True (the whole table, indexed by
+    utab entry) or False (scalar reads); pipelined kernels take none."""
+    if pipelined or not lanes:
+        return False
+    return "lds" if lanes == "lds" else True
+
+
 class ProgramKernels:
     """The specialised kernels of every pass of one encoded forward program.
 
@@ -1021,11 +1062,13 @@ class ProgramKernels:
     measuring pass drops to one (351 vs 256 us for that pass at 20 qubits).
     ``lanes`` = "lds" and ``split_pairs`` (pass_source) are the defaults: 3.48
     vs 3.67 ms per 20-qubit forward, 4.69 vs 4.89 at 24 qubits, results bit
-    for bit the same."""
+    for bit the same.  ``interleave`` (pass_source) is a default too: within
+    1% either way on these shapes, and it halves the MFMA wait states."""
 
     def __init__(self, passes, n: int, R: int, TB: int, gen_first: bool, measure: bool,
                  prefetch: bool = False, waves: int = 2, readout=None, valu_pairs: int = 0,
-                 grid: str = "full", pipeline: int = 0, lanes="lds", split_pairs: bool = True):
+                 grid: str = "full", pipeline: int = 0, lanes="lds", split_pairs: bool = True,
+                 interleave: bool = True):
         self.n, self.R, self.TB = n, R, TB
         self.pipelined = bool(pipeline) and n >= R + TB
         self.block = (1 << TB) + ((1 << pipeline) if self.pipelined else 0)
@@ -1045,11 +1088,11 @@ class ProgramKernels:
             else:
                 self.sources.append(pass_source(nm, words, n, R, TB, lpos, gen_first and i == 0,
                                                 measure and i == npass - 1, prefetch, waves, readout, valu_pairs,
-                                                lanes, split_pairs, lane_off))
+                                                lanes, split_pairs, lane_off, interleave))
             self.names.append(nm)
         self.funcs = kernels_for(self.sources, self.names)
         self.grid_cap = [None] * len(self.funcs)
-        self.lanes = (lanes if lanes == "lds" else bool(lanes)) and not self.pipelined
+        self.lanes = lane_mode(lanes, self.pipelined)
         self._lane_idx = {}
 
     def lane_index_on(self, device):
@@ -1090,7 +1133,8 @@ class AdjointKernels:
     gradients bit for bit the same."""
 
     def __init__(self, passes, n: int, R: int, TB: int, seed_first: bool, prefetch: bool = False,
-                 waves: int = 2, readout=None, grid: str = "full", lanes="lds", split_pairs: bool = True):
+                 waves: int = 2, readout=None, grid: str = "full", lanes="lds", split_pairs: bool = True,
+                 interleave: bool = True):
         self.block = 1 << TB
         self.full_grid = grid == "full"
         self.sources, self.names = [], []
@@ -1102,7 +1146,7 @@ class AdjointKernels:
             if lanes == "lds":
                 self.lane_index += adjoint_lane_entries(words, R, TB)
             self.sources.append(adjoint_source(nm, words, n, R, TB, lpos, seed_first and i == 0, prefetch, waves,
-                                               readout, lanes, split_pairs, lane_off))
+                                               readout, lanes, split_pairs, lane_off, interleave))
             self.names.append(nm)
         self.funcs = kernels_for(self.sources, self.names)
         self.grid_cap = [None] * len(self.funcs)

@@ -111,3 +111,26 @@ def test_adjoint_lane_table_variant_compiles(tmp_path, monkeypatch):
         off += len(jit.adjoint_lane_entries(w.numpy(), R, TB))
     assert off > 0 and any("ctab[" in s_ for s_ in srcs)
     assert len(jit._code_for("\n\n".join(srcs))) > 1000
+
+
+def test_lane_mode_keeps_the_compact_table_form():
+    """Kernels built with lanes="lds" index a compact per-pass block, so the
+    program must report "lds" (not merely a true value) for the caller to pack
+    the matching table."""
+    assert jit.lane_mode("lds", False) == "lds"
+    assert jit.lane_mode(True, False) is True
+    assert jit.lane_mode("lds", True) is False
+    assert jit.lane_mode(False, False) is False
+
+
+def test_lane_block_is_not_the_table_order_with_per_sample_unitaries():
+    """With per-sample unitaries between the shared ones, a pass's compact lane
+    block lists table entries out of the identity order: the case in which a
+    whole-table operand array would hand the kernels wrong operands."""
+    n, B = 12, 4
+    prog = _program(n, True)
+    R, TB = fused.choose_shape(n, grad=True)
+    pl = fused.get_plan(prog, R, TB, None, fused.folds(prog, True))
+    ws = fused.pass_words(prog, pl, B, "cpu")
+    ents = [u for w, _ns, _lp in ws.passes for u in jit.lane_entries(w.numpy(), R, TB, n < R + TB)]
+    assert ents and ents != list(range(len(ents)))
