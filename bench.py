@@ -110,8 +110,8 @@ def main():
         profile_workload(args.profile)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build targets a single MI355X "
-              "(multi-GPU sharding is designed, not built; see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU only: the engine runs on a single MI355X; "
+              "sharding a state across GPUs is not implemented.", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -1,4 +1,4 @@
-"""Module-level configuration (``src/qandle/config.py:1-3``).
+"""Module-level configuration (``src/qandle/config.py:1-75``).
 
 ``DEFAULT_MAPPING`` is read when an unnamed rotation is constructed
 (``src/qandle/operators.py:168``), exactly as in the reference.
@@ -6,6 +6,13 @@
 from . import qw_map
 
 DEFAULT_MAPPING = qw_map.tanh
+
+# Text drawings (``src/qandle/config.py:9-75``, read by ``qandle_amd.drawer``).
+DRAW_SPLITTED_PAD = 7          # blanks after each sub-circuit diagram of a split circuit
+DRAW_DASH = "─"                # the wire character
+DRAW_SHOW_VALUES = True        # False drops digits, dots, underscores, blanks and brackets from labels
+DRAW_SHIFT_LEFT = False        # True draws gates as far left as their wires allow
+DRAW_CROSS_BETWEEN_CNOT = True  # a crossing on the wires between a CNOT's control and target
 
 # Engine knobs (no counterpart in the reference).
 FUSED = True            # Circuit.forward uses the fused pass engine

@@ -8,11 +8,13 @@ pass inside a Circuit.
 """
 from __future__ import annotations
 
+import random
 import typing
 
 import torch
 
 from . import operators as op
+from . import qasm
 from . import utils
 from ._engine import plain
 
@@ -67,6 +69,9 @@ class AmplitudeEmbeddingBuilt(InputOperatorBuilt):
     def __str__(self) -> str:
         return "AmplitudeEmbedding"
 
+    def to_qasm(self) -> qasm.QasmRepresentation:
+        raise NotImplementedError("AmplitudeEmbedding is not yet supported")
+
 
 class AmplitudeEmbedding(InputOperator):
     """Amplitude embedding before build (``src/qandle/embeddings.py:68-111``)."""
@@ -85,6 +90,16 @@ class AmplitudeEmbedding(InputOperator):
 
     def __str__(self) -> str:
         return "AmplitudeEmbedding"
+
+    def to_qasm(self) -> qasm.QasmRepresentation:
+        raise NotImplementedError("AmplitudeEmbedding is not yet supported")
+
+
+def _angle_qasm(rot, qubits):
+    """One input-bound rotation per embedded qubit, named ``angle_<id>_<qubit>``
+    with a fresh id per export (``src/qandle/embeddings.py:166-171, 206-211``)."""
+    r = hash(random.random())
+    return [rot(qubit=w, name=f"angle_{r}_{w}").to_qasm() for w in qubits]
 
 
 _ROT_KIND = {"RX": 0, "RY": 1, "RZ": 2}
@@ -132,6 +147,9 @@ class AngleEmbeddingBuilt(InputOperatorBuilt):
     def __str__(self) -> str:
         return f"AngleEmbedding_{self.rotation}"
 
+    def to_qasm(self) -> typing.List[qasm.QasmRepresentation]:
+        return _angle_qasm(utils.parse_rot(self.rotation), self.qubits)
+
     def decompose(self):
         """The per-qubit named rotations this embedding is equivalent to."""
         return [self.rots(qubit=w, name=f"{self.name}_{w}") for w in self.qubits]
@@ -152,3 +170,6 @@ class AngleEmbedding(InputOperator):
 
     def __str__(self) -> str:
         return f"AngleEmbedding_{self.rotation}"
+
+    def to_qasm(self) -> typing.List[qasm.QasmRepresentation]:
+        return _angle_qasm(utils.parse_rot(self.rotation), self.qubits)

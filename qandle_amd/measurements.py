@@ -8,13 +8,18 @@ an unbatched state gives (n,), and so does a batched state with B = 1.
 """
 from __future__ import annotations
 
+import random
+import warnings
+
 import torch
 
 from . import operators as op
+from . import qasm
+from .errors import UnbuiltGateError
 from ._engine import plain
 
 __all__ = ["MeasureProbability", "MeasureProbabilityBuilt", "BuiltMeasurement", "UnbuiltMeasurement",
-           "MeasureAllAbsolute", "MeasureJointProbability", "MeasureState"]
+           "MeasureAllAbsolute", "MeasureJointProbability", "MeasureState", "MeasurementDummy"]
 
 
 def reference_shape(probs_bn: torch.Tensor) -> torch.Tensor:
@@ -22,9 +27,47 @@ def reference_shape(probs_bn: torch.Tensor) -> torch.Tensor:
     return probs_bn.transpose(0, 1).squeeze().transpose(-1, 0)
 
 
+class MeasurementDummy(op.UnbuiltOperator, torch.nn.Module):
+    """One qubit of a decomposed measurement (``src/qandle/measurements.py:30-54``):
+    a no-op on the state that keeps its place in drawings and OpenQASM."""
+
+    def __init__(self, qubit: int):
+        torch.nn.Module.__init__(self)
+        self.qubit = qubit
+
+    def __str__(self) -> str:
+        return "M"
+
+    def to_qasm(self) -> qasm.QasmRepresentation:
+        return qasm.QasmRepresentation(gate_str=f"measure q[{self.qubit}] -> c[{self.qubit}]",
+                                       qasm3_outputs=f"measured_{hash(random.random())}_{self.qubit}")
+
+    def forward(self, state: torch.Tensor) -> torch.Tensor:
+        warnings.warn("Forward on a decomposed measurement is a no-op. "
+                      "Consider acting on the resulting statevector directly")
+        return state
+
+    def build(self, num_qubits: int, **kwargs) -> "MeasurementDummy":
+        return self
+
+
 class BuiltMeasurement(op.BuiltOperator):
     def __str__(self) -> str:
         return "M"
+
+    def to_qasm(self):
+        """``measure q[w] -> c[w];`` for every qubit, each an OpenQASM 3 output
+        (``src/qandle/measurements.py:16-22``)."""
+        i = hash(random.random())
+        return [qasm.QasmRepresentation(gate_str=f"measure q[{w}] -> c[{w}]", qasm3_outputs=f"measured_{i}_{w}")
+                for w in range(self.num_qubits)]
+
+    def decompose(self):
+        """One no-op ``MeasurementDummy`` per qubit (``src/qandle/measurements.py:24-54``): a
+        decomposed circuit returns its state, as the reference's does."""
+        warnings.warn("Decomposed measurements a no-op. Consider acting on the resulting statevector directly",
+                      RuntimeWarning)
+        return [MeasurementDummy(w) for w in range(self.num_qubits)]
 
     def to_matrix(self, **kwargs):
         return torch.eye(2 ** self.num_qubits)
@@ -33,6 +76,9 @@ class BuiltMeasurement(op.BuiltOperator):
 class UnbuiltMeasurement(op.UnbuiltOperator):
     def __str__(self) -> str:
         return "M"
+
+    def to_qasm(self):
+        raise UnbuiltGateError(f"Unbuilt {self.__class__.__name__} cannot be converted to qasm.")
 
     def build(self, num_qubits: int, **kwargs) -> BuiltMeasurement:
         return BUILT_CLASS_RELATION[self.__class__](num_qubits=num_qubits, **kwargs)

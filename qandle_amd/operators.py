@@ -18,6 +18,7 @@ import torch
 
 from . import config
 from . import errors
+from . import qasm
 from . import qw_map
 from . import utils
 from ._engine import plain
@@ -67,8 +68,9 @@ class Operator(abc.ABC):
     def __repr__(self) -> str:
         return self.__str__()
 
-    def to_qasm(self):
-        raise NotImplementedError("OpenQASM export is not part of the qandle_amd engine")
+    def to_qasm(self) -> qasm.QasmRepresentation:
+        """The OpenQASM statement(s) of the operator (``src/qandle/operators.py:47-49``)."""
+        raise NotImplementedError(f"{type(self).__name__} has no OpenQASM form")
 
 
 class UnbuiltOperator(Operator, abc.ABC):
@@ -167,6 +169,17 @@ class UnbuiltParametrizedOperator(UnbuiltOperator, metaclass=AbstractNoForward):
             return f"{self.__class__.__name__}{self.qubit} ({self.remapping(self.theta).item():.2f})"
         return f"{self.__class__.__name__}{self.qubit} ({self.name})"
 
+    def to_qasm(self) -> qasm.QasmRepresentation:
+        """``rx(<angle>) q[i];``, or the input name for a named gate (``src/qandle/operators.py:186-204``)."""
+        gate = self.__class__.__name__.lower()
+        if self.named:
+            return qasm.QasmRepresentation(gate_str=gate, qubit=self.qubit, qasm3_inputs=self.name)
+        if self.theta is None:
+            raise errors.UnbuiltGateError(
+                "This gate has no parameter. Set parameter or build or set name before converting to OpenQASM2."
+            )
+        return qasm.QasmRepresentation(gate_str=gate, qubit=self.qubit, gate_value=self.remapping(self.theta).item())
+
     def build(self, num_qubits, **kwargs) -> "BuiltParametrizedOperator":
         return BUILT_CLASS_RELATION[self.__class__](
             qubit=self.qubit,
@@ -211,6 +224,13 @@ class BuiltParametrizedOperator(BuiltOperator, abc.ABC):
         if self.named:
             return f"{base} ({self.name})"
         return f"{base} ({self.remapping(self.theta).item():.2f})"
+
+    def to_qasm(self) -> qasm.QasmRepresentation:
+        """As the unbuilt gate's (``src/qandle/operators.py:250-262``)."""
+        gate = self.unbuilt_class.__name__.lower()
+        if self.named:
+            return qasm.QasmRepresentation(gate_str=gate, qubit=self.qubit, qasm3_inputs=self.name)
+        return qasm.QasmRepresentation(gate_str=gate, qubit=self.qubit, gate_value=self.remapping(self.theta).item())
 
     def angle(self, **kwargs) -> torch.Tensor:
         """phi: remap(theta) for an unnamed gate, kwargs[name] (no remap) for a named one
@@ -315,6 +335,9 @@ class CNOT(UnbuiltOperator):
     def __str__(self) -> str:
         return f"CNOT {self.c}|{self.t}"
 
+    def to_qasm(self) -> qasm.QasmRepresentation:
+        return qasm.QasmRepresentation(gate_str=f"cx q[{self.c}], q[{self.t}]")
+
     def build(self, num_qubits, **kwargs) -> "BuiltCNOT":
         return BuiltCNOT(control=self.c, target=self.t, num_qubits=num_qubits)
 
@@ -343,6 +366,9 @@ class BuiltCNOT(BuiltOperator):
     def __str__(self) -> str:
         return CNOT(self.c, self.t).__str__()
 
+    def to_qasm(self) -> qasm.QasmRepresentation:
+        return CNOT(self.c, self.t).to_qasm()
+
     def to_matrix(self, **kwargs) -> torch.Tensor:
         N = 2 ** self.num_qubits
         idx = torch.arange(N)
@@ -361,6 +387,9 @@ class U(UnbuiltOperator):
 
     def __str__(self) -> str:
         return f"U_{self.qubit}"
+
+    def to_qasm(self) -> qasm.QasmRepresentation:
+        return qasm.QasmRepresentation(gate_str="U", qubit=self.qubit)
 
     def build(self, num_qubits, **kwargs) -> "BuiltU":
         return BuiltU(qubit=self.qubit, matrix=self.matrix, num_qubits=num_qubits)
@@ -394,6 +423,14 @@ class BuiltU(BuiltOperator):
     def __str__(self) -> str:
         return f"{self.description}_{self.qubit}"
 
+    def to_qasm(self) -> qasm.QasmRepresentation:
+        """A gate definition holding the four entries to two decimals (``src/qandle/operators.py:109-124``;
+        the doubled braces are the reference's text)."""
+        m = torch.as_tensor(self.original_matrix)
+        entries = ", ".join(format(m[i, j].item(), ".2f") for i, j in ((0, 0), (0, 1), (1, 0), (1, 1)))
+        q = self.qubit
+        return qasm.QasmRepresentation(gate_str=f"gate {self.description}_{q} q[{q}] {{{{ U({entries}) }}}}", qubit=q)
+
     def forward(self, state: torch.Tensor) -> torch.Tensor:
         return _forward(self, state)
 
@@ -423,6 +460,9 @@ class Invert(UnbuiltOperator):
 
     def __str__(self) -> str:
         return f"{self.t}^-1"
+
+    def to_qasm(self) -> qasm.QasmRepresentation:
+        return qasm.QasmRepresentation(gate_str=f"{self.t}^-1")
 
     def build(self, num_qubits, **kwargs) -> "BuiltInvert":
         return BuiltInvert(target=self.t, num_qubits=num_qubits)
@@ -456,6 +496,9 @@ class BuiltInvert(BuiltOperator):
     def __str__(self) -> str:
         return Invert(self.target).__str__()
 
+    def to_qasm(self) -> qasm.QasmRepresentation:
+        return Invert(self.target).to_qasm()
+
     def to_matrix(self, **kwargs) -> torch.Tensor:
         return torch.linalg.inv(self.target.to_matrix())
 
@@ -469,6 +512,9 @@ class Controlled(UnbuiltOperator):
 
     def __str__(self) -> str:
         return f"Controlled {self.c}|{self.t}"
+
+    def to_qasm(self) -> qasm.QasmRepresentation:
+        return qasm.QasmRepresentation(gate_str=f"controlled q[{self.c}], {self.t}")
 
     def build(self, num_qubits, **kwargs) -> "BuiltControlled":
         return BuiltControlled(control=self.c, target=self.t, num_qubits=num_qubits)
@@ -504,6 +550,9 @@ class BuiltControlled(BuiltOperator):
     def __str__(self) -> str:
         return Controlled(self.c, self.t).__str__()
 
+    def to_qasm(self) -> qasm.QasmRepresentation:
+        return Controlled(self.c, self.t).to_qasm()
+
     def to_matrix(self, **kwargs) -> torch.Tensor:
         return self.t.to_matrix()
 
@@ -518,6 +567,9 @@ class CZ(UnbuiltOperator):
 
     def __str__(self) -> str:
         return f"CZ {self.c}|{self.t}"
+
+    def to_qasm(self) -> qasm.QasmRepresentation:
+        return qasm.QasmRepresentation(gate_str=f"cz q[{self.c}], q[{self.t}]")
 
     def build(self, num_qubits, **kwargs) -> "BuiltCZ":
         return BuiltCZ(control=self.c, target=self.t, num_qubits=num_qubits)
@@ -543,6 +595,9 @@ class BuiltCZ(BuiltOperator):
     def __str__(self) -> str:
         return CZ(self.c, self.t).__str__()
 
+    def to_qasm(self) -> qasm.QasmRepresentation:
+        return CZ(self.c, self.t).to_qasm()
+
     def to_matrix(self, **kwargs) -> torch.Tensor:
         idx = torch.arange(2 ** self.num_qubits)
         diag = torch.ones(2 ** self.num_qubits, dtype=torch.complex64)
@@ -558,6 +613,9 @@ class Reset(UnbuiltOperator):
 
     def __str__(self) -> str:
         return f"Reset {self.qubit}"
+
+    def to_qasm(self) -> qasm.QasmRepresentation:
+        return qasm.QasmRepresentation(gate_str="reset", qubit=self.qubit)
 
     def build(self, num_qubits, **kwargs) -> "BuiltReset":
         return BUILT_CLASS_RELATION[self.__class__](qubit=self.qubit, num_qubits=num_qubits)
@@ -589,6 +647,9 @@ class BuiltReset(BuiltOperator):
     def __str__(self) -> str:
         return Reset(self.qubit).__str__()
 
+    def to_qasm(self) -> qasm.QasmRepresentation:
+        return Reset(self.qubit).to_qasm()
+
     def to_matrix(self, **kwargs) -> torch.Tensor:
         raise ValueError("Reset gate does not have a matrix representation.")
 
@@ -602,6 +663,9 @@ class SWAP(UnbuiltOperator):
 
     def __str__(self) -> str:
         return f"Swap {self.a}|{self.b}"
+
+    def to_qasm(self) -> qasm.QasmRepresentation:
+        return qasm.QasmRepresentation(gate_str=f"swap q[{self.a}], q[{self.b}]")
 
     def build(self, num_qubits, **kwargs) -> "BuiltSWAP":
         return BuiltSWAP(a=self.a, b=self.b, num_qubits=num_qubits)
@@ -631,6 +695,9 @@ class BuiltSWAP(BuiltOperator):
 
     def __str__(self) -> str:
         return f"SWAP {self.a}|{self.b}"
+
+    def to_qasm(self) -> qasm.QasmRepresentation:
+        return SWAP(self.a, self.b).to_qasm()
 
     def to_matrix(self, **kwargs) -> torch.Tensor:
         N = 2 ** self.num_qubits

@@ -9,13 +9,17 @@ cover runs layer by layer through the operators' own kernels.
 """
 from __future__ import annotations
 
+import re
 import typing
 import warnings
 
 import torch
 
 from . import config
+from . import drawer
+from . import measurements
 from . import operators
+from . import qasm
 from . import splitter
 
 __all__ = ["Circuit", "UnsplittedCircuit", "SplittedCircuit", "Subcircuit"]
@@ -69,8 +73,6 @@ class Circuit(torch.nn.Module):
     @staticmethod
     def _used_qubits(layers) -> typing.Set[int]:
         """Infer the qubit count (``src/qandle/qcircuit.py:70-106``)."""
-        from . import measurements
-
         qubits = set()
         for layer in layers:
             if isinstance(layer, operators.CNOT):
@@ -104,6 +106,19 @@ class Circuit(torch.nn.Module):
 
     def decompose(self):
         return Circuit(layers=[], num_qubits=self.num_qubits, circuit=self.circuit.decompose())
+
+    def draw(self) -> str:
+        """Text diagram of the circuit (``src/qandle/qcircuit.py:108-110``)."""
+        return drawer.draw(self.circuit)
+
+    def to_qasm(self):
+        return self.circuit.to_qasm()
+
+    def to_openqasm2(self) -> str:
+        return qasm.convert_to_qasm(self, qasm_version=2, include_header=True)
+
+    def to_openqasm3(self) -> str:
+        return qasm.convert_to_qasm(self, qasm_version=3, include_header=True)
 
 
 class UnsplittedCircuit(torch.nn.Module):
@@ -166,6 +181,14 @@ class UnsplittedCircuit(torch.nn.Module):
                 new_layers.append(layer)
         return UnsplittedCircuit(self.num_qubits, new_layers)
 
+    def to_qasm(self):
+        """Statements of every layer that has an OpenQASM form, in order (``src/qandle/qcircuit.py:176-181``)."""
+        reps = []
+        for layer in self.layers:
+            if hasattr(layer, "to_qasm"):
+                reps.extend(layer.to_qasm())
+        return reps
+
     def split(self, **kwargs):
         if self.num_qubits == 1:
             warnings.warn("Can't split circuit with only 1 qubit")
@@ -211,6 +234,10 @@ class SplittedCircuit(UnsplittedCircuit):
             inp = sc(inp, **kwargs)
         return inp
 
+    def to_qasm(self):
+        """One statement list per sub-circuit, on parent qubits (``src/qandle/qcircuit.py:253-255``)."""
+        return [sc.to_qasm() for sc in self.subcircuits]
+
     def to_matrix(self, **kwargs):
         raise NotImplementedError("Can't get matrix representation of a splitted circuit.")
 
@@ -243,6 +270,20 @@ class Subcircuit(torch.nn.Module):
                 if phi.dim() == 1 and st.shape[0] == 1 and phi.shape[0] != 1:
                     st = st.expand(phi.shape[0], -1)
                 st = plain.RotFn.apply(st, phi, n, n - 1 - self.inverse[mod.qubit], mod.KIND)
+            elif isinstance(mod, measurements.MeasurementDummy):
+                st = mod(st)
             else:
                 raise NotImplementedError(f"{type(mod).__name__} inside a split sub-circuit")
         return st if batched or st.shape[0] != 1 else st.squeeze(0)
+
+    def to_qasm(self):
+        """The layers' statements with sub-circuit qubits renamed to parent qubits
+        (``src/qandle/qcircuit.py:297-313``): the ``qubit`` field where a statement
+        has one, else every ``q[i]`` in its text."""
+        reps = [mod.to_qasm() for mod in self.layers if hasattr(mod, "to_qasm")]
+        for r in reps:
+            if r.qubit not in ("", None):
+                r.qubit = self.inverse[r.qubit]
+            else:
+                r.gate_str = re.sub(r"q\[(\d+)\]", lambda m: f"q[{self.inverse[int(m.group(1))]}]", r.gate_str)
+        return reps

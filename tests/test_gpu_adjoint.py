@@ -1,6 +1,6 @@
 """GPU parity of the adjoint backward (fused path) against the FP64 oracle.
 
-The reference has no gradient tests (SURVEY: it only asserts requires_grad);
+The reference has no gradient tests (its tests only assert requires_grad);
 gradients are pinned by the FP64 adjoint of ``oracle/fp64.py``, which is itself
 pinned by central finite differences and parameter shift.  Bar: 1e-4 absolute
 for dL/dtheta and dL/dx with L = sum(w * MeasureProbability).

@@ -128,6 +128,59 @@ def test_split_circuit_equals_unsplit(device):
     assert torch.allclose(a, b_layers, atol=1e-6)
 
 
+@pytest.mark.parametrize("max_qubits", [2, 3, 4])
+def test_split_decomposed_su_equals_unsplit(device, max_qubits):
+    """SU's decomposition split into groups of 2, 3 and 4 qubits gives the
+    unsplit state (``test_splitter.py:84-101``), and every gate lands in one
+    sub-circuit (``:68-76``)."""
+    torch.manual_seed(4)
+    su = qandle.SU(qubits=list(range(5)), reps=2, rotations=["rx", "ry"])
+    c = qandle.Circuit(layers=su.decompose()).decompose()
+    s = c.split(max_qubits=max_qubits)
+    assert len(c.circuit.layers) == sum(len(sc.layers) for sc in s.circuit.subcircuits)
+    c, s = c.to(device), s.to(device)
+    gen = torch.Generator().manual_seed(5)
+    inp = torch.rand(2 ** 5, dtype=torch.cfloat, generator=gen)
+    inp = (inp / torch.linalg.norm(inp)).to(device)
+    with torch.no_grad():
+        a, b = c(inp), s(inp)
+    ops = []
+    for m in c.circuit.layers:
+        if isinstance(m, qandle.operators.BuiltCNOT):
+            ops.append(("cnot", m.c, m.t))
+        else:
+            kind = {qandle.operators.BuiltRX: "rx", qandle.operators.BuiltRY: "ry"}[type(m)]
+            ops.append((kind, m.qubit, float(m.remapping(m.theta).detach().reshape(-1)[0])))
+    ref = fp64.simulate(5, ops, inp.cpu().numpy().astype(np.complex128)[None, :])[0]
+    assert np.abs(a.cpu().numpy() - ref).max() < ATOL
+    assert torch.allclose(a, b, atol=1e-6)
+
+
+def test_decomposed_measurement_returns_the_state(device):
+    """A decomposed circuit's measurement is a no-op (``measurements.py:24-54``)."""
+    import warnings
+
+    torch.manual_seed(6)
+    n = 7
+    body = [qandle.RY(q, remapping=None) for q in range(n)] + [qandle.CNOT(q, (q + 1) % n) for q in range(n)]
+    circ = qandle.Circuit(body + [qandle.MeasureProbability()], num_qubits=n)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        dec = circ.decompose().to(device)
+        gen = torch.Generator().manual_seed(7)
+        inp = torch.randn(3, 2 ** n, dtype=torch.cfloat, generator=gen)
+        inp = (inp / torch.linalg.norm(inp, dim=-1, keepdim=True)).to(device)
+        with torch.no_grad():
+            out = dec(inp)
+            plain = dec.circuit.forward_layers(inp)
+    assert out.shape == (3, 2 ** n) and out.dtype == torch.complex64
+    assert (out - plain).abs().max().item() < ATOL
+    ops = [("ry", m.qubit, float(m.theta.detach().reshape(-1)[0])) for m in circ.circuit.layers[:n]]
+    ops += [("cnot", q, (q + 1) % n) for q in range(n)]
+    ref = fp64.simulate(n, ops, inp.cpu().numpy().astype(np.complex128))
+    assert np.abs(out.cpu().numpy() - ref).max() < ATOL
+
+
 @pytest.mark.parametrize("R,TB", [(5, 7), (5, 6), (4, 8), (3, 8)])
 def test_kernel_variants(device, R, TB):
     w = workloads.rxryrz_ladder(n=15, B=3, depth=2, seed=7)
