@@ -13,6 +13,12 @@ class Ansatz(op.Operator, abc.ABC):
     def decompose(self) -> typing.List[op.Operator]:
         """The gates the ansatz is made of."""
 
+    def to_qasm(self):
+        """The statements of its gates in order (``src/qandle/ansaetze/specialunitary.py:38-39, 113-114``,
+        ``twolocal.py:43-44, 108-114``, ``stronglyentangling.py:126-127``,
+        ``stronglyentangling_budget.py:83-84``)."""
+        return [g.to_qasm() for g in self.decompose()]
+
 
 class UnbuiltAnsatz(Ansatz, op.UnbuiltOperator, abc.ABC):
     """An ansatz exposed to the user; built into a :class:`BuiltAnsatz`."""

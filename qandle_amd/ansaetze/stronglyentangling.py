@@ -54,6 +54,9 @@ class StronglyEntanglingLayer(UnbuiltAnsatz):
     def __str__(self) -> str:
         return "SEL"
 
+    def to_qasm(self):
+        raise NotImplementedError("TODO")  # the reference's unbuilt SEL has no export either (:57-58)
+
     def decompose(self) -> typing.List[op.UnbuiltOperator]:
         layers = []
         for d in range(self.depth):

@@ -155,3 +155,22 @@ def test_decomposed_measurement_is_a_no_op_program():
     prog = fused.lower_circuit(d.circuit)
     assert prog is not None and not prog.measure  # the state comes back, as from the reference's decomposition
     assert [r.gate_str for r in d.to_qasm()[-4:]] == [f"measure q[{w}] -> c[{w}]" for w in range(4)]
+
+
+def test_ansatz_exports_are_their_gates_in_order():
+    torch.manual_seed(0)
+    n = 3
+    sel = qandle.StronglyEntanglingLayer(qubits=list(range(n)), depth=2).build(num_qubits=n)
+    su = qandle.SU(qubits=list(range(n)), reps=1).build(num_qubits=n)
+    tl = qandle.TwoLocal(depth=2).build(num_qubits=n)
+    for m in (sel, su, tl):
+        reps = m.to_qasm()
+        assert [r.to_string() for r in reps] == [g.to_qasm().to_string() for g in m.decompose()]
+    # TwoLocal: each depth's RY layer, then the CNOT chain q -> q + 1 (twolocal.py:108-114)
+    txt = [r.to_string() for r in tl.to_qasm()]
+    assert [t.split("(")[0] for t in txt] == (["ry"] * n + ["cx q[0], q[1];", "cx q[1], q[2];"]) * 2
+    with pytest.raises(NotImplementedError):
+        qandle.StronglyEntanglingLayer(qubits=[0, 1], depth=1).to_qasm()  # as the reference's unbuilt SEL
+    with pytest.raises(qandle.UnbuiltGateError):
+        # the budget ansatz exports its unbuilt gates, which carry no angle (stronglyentangling_budget.py:83-84)
+        qandle.StronglyEntanglingLayerBudget(n, param_budget=4).to_qasm()
