@@ -82,10 +82,15 @@ def profile_workload(mode, reps=2):
 
 
 def executed_plan(prog, R, TB):
-    """The plan a forward without state output runs: the compiled kernels fold
-    trailing CNOTs into the readout (fused.readout_fold)."""
+    """The plan a forward without state output runs from a product state: the
+    rotations ahead of every CNOT on their qubit go into the qubits' factors
+    (fused.leading_rotations) and the compiled kernels fold trailing CNOTs into
+    the readout (fused.readout_fold)."""
+    from qandle_amd import config
     from qandle_amd._engine import fused
 
+    if config.ABSORB_LEADING_ROTATIONS and fused.leading_rotations(prog) is not None:
+        prog = fused.leading_rotations(prog)[0]
     return fused.get_plan(prog, R, TB, None, fused.folds(prog, fused.use_jit(prog.n, R, TB)))
 
 

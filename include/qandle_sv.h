@@ -208,6 +208,20 @@ int qs_product_factors(float *efac, const float *x, int64_t B, int n,
                        const int32_t *bits, int d, int rot, int64_t x_stride,
                        void *stream);
 
+/* Product-state factors with single-qubit unitaries applied:
+ * out[b, pos] = U efac[b, pos] for every position with unit[pos] >= 0 (host
+ * arrays of n entries), U the 2x2 utab entry unit[pos] (+ b when span[pos] is
+ * B; span 1 is one entry for the batch), other positions copied.  utab has nu
+ * entries (qs_compose_tables).  A circuit's rotations that precede every CNOT
+ * on their qubit act on an unentangled qubit, so they are applied here to its
+ * factor instead of to the 2^n amplitudes (the rotations of
+ * src/qandle/operators.py:215-305 that follow src/qandle/embeddings.py:148-164
+ * directly); no counterpart in the reference.  Returns QS_EINVAL when an entry
+ * lies outside the table. */
+int qs_factor_apply(float *out, const float *efac, const float *utab, int64_t nu,
+                    const int32_t *unit, const int32_t *span, int64_t B, int n,
+                    void *stream);
+
 /* Fused per-bit unitaries and gradient generators for a program's words:
  * `recipes` (device int32, record r at offsets[r]) lists, per table entry,
  * u_index, span (batch size of a per-sample entry, else 1), nconst, q_index

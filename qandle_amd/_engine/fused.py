@@ -630,6 +630,64 @@ def product_factors(prog: Program, inp: Inputs) -> torch.Tensor:
     return efac
 
 
+def leading_rotations(prog: Program):
+    """(program of the remaining gates, rotations by bit position) or None.
+
+    A rotation on a qubit that no CNOT has touched yet acts on an unentangled
+    qubit of a product state, so it can be applied to that qubit's 2-vector
+    factor (qs_factor_apply) instead of to all 2^n amplitudes: the first
+    layer of an embedding circuit then costs B * n small products.  The
+    remaining program keeps the slots (and so the coefficient layout) of the
+    full one.  None when nothing is absorbed or nothing would remain."""
+    hit = prog.words_cache.get("leading", False)
+    if hit is not False:
+        return hit
+    hit = None
+    if prog.init[0] in ("embed", "input"):
+        touched, keep, per_pos = set(), [], {}
+        for g in prog.gates:
+            if g.kind in (PL.RX, PL.RY, PL.RZ) and g.pos not in touched:
+                per_pos.setdefault(g.pos, []).append(g)
+                continue
+            touched.update(g.positions)
+            keep.append(g)
+        if per_pos and keep:
+            hit = (Program(prog.n, keep, prog.slots, prog.init, prog.measure, prog.post), per_pos)
+    prog.words_cache["leading"] = hit
+    return hit
+
+
+def absorb_factors(prog: Program, per_pos, efac: torch.Tensor, cs: torch.Tensor, B: int) -> torch.Tensor:
+    """Product-state factors with the leading rotations applied (leading_rotations):
+    their unitaries are composed per bit position on the device (one recipe
+    per position, per sample when an input angle is among them) and applied by
+    qs_factor_apply."""
+    key = ("leading_tables", B, str(efac.device))
+    hit = prog.words_cache.get(key)
+    if hit is None:
+        index, per_sample, _, _ = coefficient_layout(prog, B)
+        tables = PL.TableBuilder(B)
+        unit = np.full(prog.n, -1, dtype=np.int32)
+        span = np.ones(prog.n, dtype=np.int32)
+        for pos in sorted(per_pos):
+            u, ps, _ = tables.add([(g.kind, index[g.slot], per_sample[g.slot]) for g in per_pos[pos]])
+            unit[pos], span[pos] = u, (B if ps else 1)
+        recipes = torch.from_numpy(tables.recipes()).to(efac.device)
+        offsets = torch.tensor(tables.offsets, dtype=torch.int32, device=efac.device)
+        hit = (recipes, offsets, tables.count, tables.nu, unit, span)
+        prog.words_cache[key] = hit
+    recipes, offsets, nrec, nu, unit, span = hit
+    lib = L.load()
+    st = L.stream_of(efac)
+    utab = torch.empty(max(nu, 1) * 8, dtype=torch.float32, device=efac.device)
+    L.check(lib.qs_compose_tables(L.ptr(utab), None, L.ptr(recipes), L.ptr(offsets), nrec, L.ptr(cs), st),
+            "qs_compose_tables")
+    out = torch.empty_like(efac)
+    L.check(lib.qs_factor_apply(L.ptr(out), L.ptr(efac), L.ptr(utab), nu, L.i32_array(unit), L.i32_array(span),
+                                B, prog.n, st), "qs_factor_apply")
+    return out
+
+
 def tiles_of(B: int, n: int, K: int) -> int:
     """Tiles of one pass (as qs_fused_pass counts them)."""
     if n >= K:
@@ -672,7 +730,23 @@ def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
     ``fold``: fold trailing CNOTs into the readout of a measured program
     (readout_fold; compiled kernels only), so that the state left behind is
     the one before those CNOTs; None folds when the state is not wanted."""
+    from .. import config
+
     n, B = prog.n, inp.B
+    src = None
+    if prog.init[0] in ("input", "amplitude") and inp.state is not None:
+        s = inp.state.to(torch.complex64).resolve_conj()
+        if s.dim() == 1:
+            s = s.unsqueeze(0)
+        if s.shape[0] != B:
+            s = s.expand(B, -1)
+        src = s.contiguous()
+    if src is None and efac is not None and checkpoints is None and getattr(config, "ABSORB_LEADING_ROTATIONS", True):
+        # the passes then run the remaining gates from the updated factors (same slots, same cs)
+        lead = leading_rotations(prog)
+        if lead is not None:
+            efac = absorb_factors(prog, lead[1], efac, cs, B)
+            prog = lead[0]
     jit_on = use_jit(n, R, TB) if specialise is None else specialise
     fold = folds(prog, jit_on) and (not want_state if fold is None else bool(fold))
     pl = get_plan(prog, R, TB, cm, fold)
@@ -682,14 +756,6 @@ def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
     st = L.stream_of(cs)
     utab, _ = compose_tables(ws, cs, generators=False)
     psi = torch.empty(B, 2 ** n, dtype=torch.complex64, device=inp.device)
-    src = None
-    if prog.init[0] in ("input", "amplitude") and inp.state is not None:
-        s = inp.state.to(torch.complex64).resolve_conj()
-        if s.dim() == 1:
-            s = s.unsqueeze(0)
-        if s.shape[0] != B:
-            s = s.expand(B, -1)
-        src = s.contiguous()
     mpart = probs = None
     if prog.measure:
         wb = (lib.qs_readout_work_bytes if fold else lib.qs_fused_measure_work_bytes)(B, n, R, TB)

@@ -53,6 +53,7 @@ SIGNATURES = {
     "qs_fused_supported": (I32, [I32, I32]),
     "qs_layout_supported": (I32, [I32, I32]),
     "qs_product_factors": (I32, [P, P, I64, I32, P, I32, I32, I64, P]),
+    "qs_factor_apply": (I32, [P, P, P, I64, P, P, I64, I32, P]),
     "qs_fused_pass": (I32, [P, P, I64, I32, I32, I32, P, P, I32, P, P, P, P, I32, P]),
     "qs_jit_compile": (I32, [ctypes.c_char_p, ctypes.c_char_p, P, I32, P]),
     "qs_jit_code_size": (I64, [P]),

@@ -102,11 +102,13 @@ def _gate_cost(g: Gate, in_regs: bool, cm: CostModel) -> float:
     return cm.rot
 
 
-def _rank_positions(gates, pending, universe):
-    """Positions of ``universe`` ordered by the first pending gate that needs them."""
+def _rank_positions(gates, pending, universe, touch: bool = False):
+    """Positions of ``universe`` ordered by the first pending gate that needs
+    them (``touch``: that touches them, a CNOT's control included)."""
     first = {}
     for idx in pending:
-        for p in gates[idx].need:
+        g = gates[idx]
+        for p in (g.positions if touch else g.need):
             if p in universe and p not in first:
                 first[p] = idx
     return sorted(first, key=lambda p: first[p])
@@ -141,7 +143,9 @@ def _pack(gates, idxs, L, R, cm):
     pending = list(idxs)
     Lset = set(L)
     while True:
-        ranked = _rank_positions(gates, pending, Lset)
+        # register positions: the ones the next pending gates touch, so that a
+        # CNOT whose control is also in registers is a free register swap
+        ranked = _rank_positions(gates, pending, Lset, touch=True)
         P = ranked[:R]
         for p in L:  # fill with unused local positions
             if len(P) == R:
@@ -161,37 +165,63 @@ def _pack(gates, idxs, L, R, cm):
 
 
 def plan(n: int, gates: typing.List[Gate], R: int, TB: int, cm: CostModel = None) -> Plan:
+    """Passes (local position sets) and their sub-passes (register sets) for
+    ``gates``.  Two greedy schedules are built, one choosing each pass's local
+    positions by the positions the next pending gates touch, one taking per
+    pass whichever of that and the positions they need (a CNOT's target only)
+    runs more work; the plan with the lower score (4 per pass, a pass costing
+    about four sub-passes of HBM time, plus 1 per sub-pass) is returned."""
     cm = cm or CostModel()
     K = R + TB
     if R > n:
         raise ValueError("register bits exceed qubit count")
-    pending = list(range(len(gates)))
-    passes = []
     if n < K:
         # the tile holds whole samples: one pass runs every gate
+        pending = list(range(len(gates)))
         L = list(range(K))
         subs = _pack(gates, pending, list(range(n)), R, cm) if pending else [SubPass(list(range(R)), [])]
         return Plan(n, R, TB, [Pass(L, subs, pending)])
+    best = None
+    for modes in ((True,), (False, True)):
+        pl = _plan_passes(n, gates, R, TB, cm, modes)
+        score = 4 * len(pl.passes) + sum(len(p.subs) for p in pl.passes)
+        if best is None or score < best[0]:
+            best = (score, pl)
+    return best[1]
+
+
+def _plan_passes(n: int, gates, R: int, TB: int, cm: CostModel, modes) -> Plan:
+    """One greedy schedule; ``modes``: the local-set rankings tried per pass
+    (True: by touch, False: by need), the one that runs more work is taken."""
+    K = R + TB
+    pending = list(range(len(gates)))
+    passes = []
     low = list(range(min(cm.min_low, K - 1)))  # always leave room for a needed position
+    budget = cm.budget - cm.subpass * 2
     while True:
-        ranked = [p for p in _rank_positions(gates, pending, set(range(n))) if p not in low]
-        L = list(low) + ranked[: K - len(low)]
-        for p in range(n):  # fill with the lowest unused positions
-            if len(L) == K:
-                break
-            if p not in L:
-                L.append(p)
-        L = sorted(L)
-        Lset = set(L)
+        best = None
+        for touch in modes:
+            ranked = [p for p in _rank_positions(gates, pending, set(range(n)), touch) if p not in low]
+            L = list(low) + ranked[: K - len(low)]
+            for p in range(n):  # fill with the lowest unused positions
+                if len(L) == K:
+                    break
+                if p not in L:
+                    L.append(p)
+            L = sorted(L)
+            Lset = set(L)
 
-        def ok(g):
-            return all(p in Lset for p in g.need)
+            def ok(g, Lset=Lset):
+                return all(p in Lset for p in g.need)
 
-        def cost(g):
-            return _gate_cost(g, g.kind != RZ or g.pos in Lset, cm)
+            def cost(g, Lset=Lset):
+                return _gate_cost(g, g.kind != RZ or g.pos in Lset, cm)
 
-        budget = cm.budget - cm.subpass * 2
-        done = _scan(gates, pending, ok, budget, cost) if pending else []
+            done = _scan(gates, pending, ok, budget, cost) if pending else []
+            work = sum(cost(gates[i]) for i in done)
+            if best is None or work > best[0]:
+                best = (work, L, done)
+        _, L, done = best
         subs = _pack(gates, done, L, R, cm) if done else [SubPass(L[:R], [])]
         passes.append(Pass(L, subs, done))
         ds = set(done)

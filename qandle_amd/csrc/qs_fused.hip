@@ -474,6 +474,33 @@ __global__ void k_product_factors(float *efac, const float *x, int64_t B, int n,
   }
 }
 
+// Single-qubit unitaries applied to product-state factors: out[b][p] = U f[b][p]
+// with U the utab entry a.slot_of_bit[p] (+ b when bit p of per_sample is set);
+// positions without an entry are copied.
+__global__ void k_factor_apply(float *out, const float *efac, const float *utab, int64_t B, int n,
+                               FactorArgs a, uint64_t per_sample) {
+  const int64_t count = B * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = e / n;
+    const int pos = (int)(e % n);
+    const float2 *f = reinterpret_cast<const float2 *>(efac) + e * 2;
+    float2 g0 = f[0], g1 = f[1];
+    const int u = a.slot_of_bit[pos];
+    if (u >= 0) {
+      const float *m = utab + 8 * ((int64_t)u + (((per_sample >> pos) & 1) ? b : 0));
+      const float2 f0 = g0, f1 = g1;
+      g0 = make_float2(m[0] * f0.x - m[1] * f0.y + m[2] * f1.x - m[3] * f1.y,
+                       m[0] * f0.y + m[1] * f0.x + m[2] * f1.y + m[3] * f1.x);
+      g1 = make_float2(m[4] * f0.x - m[5] * f0.y + m[6] * f1.x - m[7] * f1.y,
+                       m[4] * f0.y + m[5] * f0.x + m[6] * f1.y + m[7] * f1.x);
+    }
+    float2 *o = reinterpret_cast<float2 *>(out) + e * 2;
+    o[0] = g0;
+    o[1] = g1;
+  }
+}
+
 // Factors of many product states at once: entry e (one bit position of one
 // sample of one circuit) reads x[entries[2e]] (or is |0> when that index is
 // -1) with rotation entries[2e + 1].
@@ -881,6 +908,28 @@ int qs_product_factors(float *efac, const float *x, int64_t B, int n, const int3
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(k_product_factors, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
                      efac, d > 0 ? x : nullptr, B, n, x_stride, a);
+  return launch_status();
+}
+
+int qs_factor_apply(float *out, const float *efac, const float *utab, int64_t nu, const int32_t *unit,
+                    const int32_t *span, int64_t B, int n, void *stream) {
+  if (B < 1 || n < 1 || n > QS_MAX_QUBITS || !out || !efac || !unit || !span || nu < 0) return QS_EINVAL;
+  FactorArgs a;
+  uint64_t per_sample = 0;
+  for (int p = 0; p < n; ++p) {
+    const int32_t u = unit[p];
+    a.slot_of_bit[p] = -1;
+    if (u < 0) continue;
+    if (!utab || (span[p] != 1 && span[p] != B) || (int64_t)u + span[p] > nu) return QS_EINVAL;
+    a.slot_of_bit[p] = u;
+    if (span[p] == B && B > 1) per_sample |= uint64_t(1) << p;
+  }
+  a.rot = 0;
+  const int64_t count = B * n;
+  int64_t blocks = (count + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_factor_apply, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, out, efac,
+                     utab, B, n, a, per_sample);
   return launch_status();
 }
 

@@ -160,3 +160,47 @@ def test_segmenter_fuses_runs_and_splits_at_cnots():
     assert phases == [5]
     assert segs[0].bits == {0: [0, 1]} and segs[0].cnots == [2]
     assert segs[1].bits == {1: [3], 0: [4]} and segs[1].cnots == []
+
+
+def _rot2(kind, c, s):
+    if kind == PL.RX:
+        return np.array([[c, -1j * s], [-1j * s, c]])
+    if kind == PL.RY:
+        return np.array([[c, -s], [s, c]])
+    return np.array([[c - 1j * s, 0], [0, c + 1j * s]])
+
+
+@pytest.mark.parametrize("mode", ["ladder", "random"])
+def test_leading_rotations_go_into_the_product_factors(mode):
+    """Rotations ahead of every CNOT on their qubit (control or target) act on
+    an unentangled qubit: applied to its factor, the remaining program gives
+    the oracle's state; a rotation after a CNOT that only controls its qubit
+    stays in the program."""
+    from qandle_amd._engine import fused
+
+    rng = np.random.default_rng(11 if mode == "ladder" else 12)
+    n, B = 8, 3
+    ops = random_ops(rng, n, 3, B, mode)
+    gates, cs, efac = lower(n, ops, B)
+    prog = fused.Program(n, gates, [fused.Slot("value")] * len(cs), ("embed", 0, list(range(n)), "x"), True)
+    rest, per_pos = fused.leading_rotations(prog)
+    assert sum(len(v) for v in per_pos.values()) + len(rest.gates) == len(gates)
+    touched = set()
+    for g in gates:
+        if g.kind == PL.CNOT:
+            touched.update(g.positions)
+        elif g.pos not in touched:
+            assert any(h is g for h in per_pos[g.pos])
+    ef = efac.copy()
+    for pos, gs in per_pos.items():
+        for g in gs:
+            U = _rot2(g.kind, *cs[g.slot])
+            ef[:, pos] = ef[:, pos] @ U.T
+    state, probs, _ = emulate_program(rest.gates, n, B, 3, 4, cs, ef, cm=PL.CostModel(budget=40.0, min_low=2))
+    ref = fp64.simulate(n, ops)
+    np.testing.assert_allclose(state, ref, atol=1e-12)
+    # the control side counts: RX on a CNOT's control after that CNOT does not commute with it
+    g2 = [PL.Gate(PL.CNOT, 0, 1), PL.Gate(PL.RX, 1, -1, 0), PL.Gate(PL.RY, 2, -1, 1)]
+    p2 = fused.Program(3, g2, [fused.Slot("value")] * 2, ("input",), True)
+    rest2, per2 = fused.leading_rotations(p2)
+    assert [g.kind for g in rest2.gates] == [PL.CNOT, PL.RX] and list(per2) == [2]
