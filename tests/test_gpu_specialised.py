@@ -216,3 +216,51 @@ def test_pass_kernel_variants_are_bit_identical(device, specialise, opts):
     assert torch.equal(s0, s1) and torch.equal(p0, p1)
     ref = fp64.measure_prob(fp64.simulate(n, w.oracle_ops()), n)
     assert np.abs(p1.cpu().numpy() - ref).max() < ATOL
+
+
+@pytest.mark.parametrize("embed", [True, False])
+@pytest.mark.parametrize("n,B", [(9, 5), (14, 3)])
+def test_leading_rotations_in_factors_vs_fp64(device, specialise, embed, n, B):
+    """Forward without gradients from a product state (an AngleEmbedding, or
+    |0..0>): the rotations ahead of every CNOT on their qubit, a per-sample
+    named one among them, go into the qubits' factors (qs_factor_apply); the
+    result equals the FP64 oracle and the same forward with them in the passes."""
+    gen = torch.Generator().manual_seed(n * 7 + B)
+    x = torch.rand(B, n, generator=gen) * 3
+    r = torch.rand(B, generator=gen) * 3
+    th = (torch.rand(3 * n + 4, generator=gen) * 6).tolist()
+    layers = [qandle.AngleEmbedding(name="x", qubits=list(range(n)), rotation="ry")] if embed else []
+    layers += [qandle.RX(q, theta=th[q], remapping=None) for q in range(n)]
+    layers += [qandle.RZ(0, name="r", remapping=None), qandle.RY(n - 1, theta=th[n], remapping=None)]
+    layers += [qandle.CNOT(q, q + 1) for q in range(n - 1)]
+    layers += [qandle.RY(q, theta=th[n + 1 + q], remapping=None) for q in range(n)]
+    layers += [qandle.RX(0, theta=th[2 * n + 1], remapping=None), qandle.CNOT(n - 1, 0), qandle.MeasureProbability()]
+    circ = qandle.Circuit(layers, num_qubits=n).to(device)
+    from qandle_amd._engine import fused
+
+    prog = fused.lower_circuit(circ.circuit)
+    lead = fused.leading_rotations(prog)
+    assert lead is not None and sum(len(v) for v in lead[1].values()) == n + 2
+    kw = {"r": r.to(device)}
+    if embed:
+        kw["x"] = x.to(device)
+    old = config.ABSORB_LEADING_ROTATIONS
+    try:
+        outs = {}
+        for absorb in (True, False):
+            config.ABSORB_LEADING_ROTATIONS = absorb
+            with torch.no_grad():
+                outs[absorb] = both(lambda: circ(**kw).cpu().numpy())
+    finally:
+        config.ABSORB_LEADING_ROTATIONS = old
+    ops = [("embed", "y", list(range(n)), x.double().numpy())] if embed else []
+    ops += [("rx", q, th[q]) for q in range(n)]
+    ops += [("rz", 0, r.double().numpy()), ("ry", n - 1, th[n])]
+    ops += [("cnot", q, q + 1) for q in range(n - 1)]
+    ops += [("ry", q, th[n + 1 + q]) for q in range(n)]
+    ops += [("rx", 0, th[2 * n + 1]), ("cnot", n - 1, 0)]
+    ref = fp64.measure_prob(fp64.simulate(n, ops), n)
+    for absorb in (True, False):
+        for P in outs[absorb]:
+            assert np.abs(P - ref).max() < ATOL, (absorb, np.abs(P - ref).max())
+    assert np.abs(outs[True][1] - outs[False][1]).max() < 1e-6
