@@ -535,7 +535,7 @@ def compose_tables(ws: WordSet, cs: torch.Tensor, generators: bool = True):
 
 def _encode(p, gates, pl, index, per_sample, tables, rows=None):
     w = PL.encode_pass(p, gates, pl.R, pl.TB, index, per_sample, tables,
-                          reverse=rows is not None, row_of=rows)
+                       reverse=rows is not None, row_of=rows, keep_waves=pl.n >= pl.R + pl.TB)
     return w, len(p.subs), np.asarray(p.L, dtype=np.int32)
 
 

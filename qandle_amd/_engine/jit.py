@@ -206,7 +206,8 @@ def lane_entries(words, R: int, TB: int, multi: bool):
 
 def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, measure: bool,
                 prefetch: bool = False, waves: int = None, readout=None, valu_pairs: int = 0,
-                lanes=False, split_pairs: bool = False, lane_off: int = 0, interleave: bool = False) -> str:
+                lanes=False, split_pairs: bool = False, lane_off: int = 0, interleave: bool = False,
+                wave_sync: bool = False) -> str:
     """HIP source of one specialised forward pass kernel (extern "C" ``name``).
 
     ``prefetch`` keeps the next tile's HBM loads in flight in registers while
@@ -228,7 +229,11 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
     16-byte access with selects.  ``interleave``: the matrix-core updates of a
     lane-table unitary run two amplitude pairs at a time with their MFMA
     chains interleaved (mfma_pair2), which fills the dependent-accumulation
-    wait states; same arithmetic per pair, so results are bit for bit equal."""
+    wait states; same arithmetic per pair, so results are bit for bit equal.
+    ``wave_sync``: between two sub-passes that give the thread-index bits
+    above the lane bits (the wave index) the same tile bits, each wave works
+    on its own part of the tile and a wave-level fence replaces the block
+    barrier (wave_local)."""
     K = R + TB
     T, NR = 1 << TB, 1 << R
     multi = n < K
@@ -306,7 +311,8 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
     e(f"    float2 v[{NR}];")
     for s, sub in enumerate(subs):
         _sub_pass(e, sub, s, len(subs), n, R, TB, multi, gen and s == 0, measure and s == len(subs) - 1, readout,
-                  valu_pairs, lanes=lane_k if lanes == "lds" else lanes, interleave=interleave)
+                  valu_pairs, lanes=lane_k if lanes == "lds" else lanes, interleave=interleave,
+                  wave_sync=wave_sync and s + 1 < len(subs) and wave_local(sub, subs[s + 1], TB))
     e("    if (A.store) {")
     e("      const int tz = pinv(tid);")
     for it in range(NR // 2):
@@ -427,8 +433,15 @@ def pass_source_pipelined(name: str, words, n: int, R: int, TB: int, lpos, gen: 
     return "\n".join(o)
 
 
+def wave_local(a, b, TB: int) -> bool:
+    """True when sub-passes a and b (decoded) map the wave-index bits (thread
+    bits 6 and up) to the same tile bits: the tile elements a wave holds are
+    then the same set in both, so passing them through LDS needs no block barrier."""
+    return list(a.G[6:TB]) == list(b.G[6:TB])
+
+
 def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu_pairs=0, pipe=None, lanes=False,
-              interleave=False):
+              interleave=False, wave_sync=False):
     """One sub-pass.  ``pipe`` (pipelined kernels): a dict; the tile is
     ``buf[boff + i]``, no closing barrier is emitted, and a measurement's
     cross-wave finish is left in ``pipe["finish"]`` for after the caller's
@@ -570,7 +583,12 @@ def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu
         e(f"      tile[lb ^ {swj[j]}] = v[{phys[j]}];")
     if measure:  # after the tile writes: the amplitudes' registers are free for the partials
         _measure(e, sub, phys, n, R, TB, multi, readout, pipe)
-    if pipe is None:
+    if pipe is None and wave_sync:
+        # the next sub-pass keeps this one's wave-index bits: every wave reads back only the
+        # quarter of the tile it wrote, and a wave's LDS operations complete in order
+        e("      __builtin_amdgcn_fence(__ATOMIC_RELEASE, \"wavefront\"); __builtin_amdgcn_wave_barrier();")
+        e("      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, \"wavefront\");")
+    elif pipe is None:
         e("      __syncthreads();")
     e("    }")
 
@@ -1068,7 +1086,7 @@ class ProgramKernels:
     def __init__(self, passes, n: int, R: int, TB: int, gen_first: bool, measure: bool,
                  prefetch: bool = False, waves: int = 2, readout=None, valu_pairs: int = 0,
                  grid: str = "full", pipeline: int = 0, lanes="lds", split_pairs: bool = True,
-                 interleave: bool = True):
+                 interleave: bool = True, wave_sync: bool = True):
         self.n, self.R, self.TB = n, R, TB
         self.pipelined = bool(pipeline) and n >= R + TB
         self.block = (1 << TB) + ((1 << pipeline) if self.pipelined else 0)
@@ -1088,7 +1106,7 @@ class ProgramKernels:
             else:
                 self.sources.append(pass_source(nm, words, n, R, TB, lpos, gen_first and i == 0,
                                                 measure and i == npass - 1, prefetch, waves, readout, valu_pairs,
-                                                lanes, split_pairs, lane_off, interleave))
+                                                lanes, split_pairs, lane_off, interleave, wave_sync))
             self.names.append(nm)
         self.funcs = kernels_for(self.sources, self.names)
         self.grid_cap = [None] * len(self.funcs)

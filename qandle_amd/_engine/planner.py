@@ -334,8 +334,33 @@ class TableBuilder:
         return np.asarray(self.words if self.words else [0], dtype=np.int32)
 
 
+def thread_bits(K: int, TB: int, regs_list, keep_waves: bool):
+    """Tile bits of the thread index of each sub-pass, lowest thread bit first.
+
+    The bits that are not registers, ascending; with ``keep_waves`` the top
+    thread bits (6 and up: the wave index of a 64-lane wave) are chosen so
+    that consecutive sub-passes share them where their registers allow: each
+    wave then holds the same part of the tile in both, and the kernels pass it
+    on with a wave-level fence instead of a block barrier."""
+    free = [[j for j in range(K) if j not in regs] for regs in regs_list]
+    nw = TB - 6
+    if not keep_waves or nw <= 0:
+        return free
+    out, prev = [], None
+    for s, fr in enumerate(free):
+        nxt = set(free[s + 1]) if s + 1 < len(free) else set(fr)
+        if prev is not None and all(j in fr for j in prev):
+            waves = prev
+        else:
+            both = [j for j in reversed(fr) if j in nxt]
+            waves = (both + [j for j in reversed(fr) if j not in both])[:nw][::-1]
+        out.append([j for j in fr if j not in waves] + list(waves))
+        prev = waves
+    return out
+
+
 def encode_pass(p: Pass, gates, R: int, TB: int, coef_index, per_sample, tables: TableBuilder,
-                reverse: bool = False, row_of=None, B: int = None) -> np.ndarray:
+                reverse: bool = False, row_of=None, B: int = None, keep_waves: bool = False) -> np.ndarray:
     """int32 words of one pass in the layout of qs_program.h.
 
     ``coef_index(slot)`` is the float2 index of a rotation's (cos, sin) entry
@@ -345,16 +370,18 @@ def encode_pass(p: Pass, gates, R: int, TB: int, coef_index, per_sample, tables:
     per register bit) followed by their CNOT words.  ``reverse`` encodes for
     the adjoint kernel: sub-passes, segments and CNOTs in reverse order, each
     segment's CNOT words before its unitary entries (the order they are undone
-    in), and every rotation's gradient row ``row_of(gate)``."""
+    in), and every rotation's gradient row ``row_of(gate)``.  ``keep_waves``:
+    thread_bits keeps the wave-index bits across sub-passes where it can (for
+    tiles that hold part of one sample)."""
     K = R + TB
     L = p.L
     loc = {pos: j for j, pos in enumerate(L)}
     rows = (lambda gi: row_of(gi)) if (reverse and row_of is not None) else (lambda gi: -1)
     words = []
     subs = list(reversed(p.subs)) if reverse else p.subs
-    for sp in subs:
+    Gs = thread_bits(K, TB, [[loc[q] for q in sp.P] for sp in subs], keep_waves)
+    for sp, G in zip(subs, Gs):
         Pl = [loc[q] for q in sp.P]
-        G = [j for j in range(K) if j not in Pl]
         phases, segs = segment_subpass(sp, gates)
         if reverse:
             segs = list(reversed(segs))
