@@ -697,13 +697,15 @@ def adjoint_lane_entries(words, R: int, TB: int):
 
 def adjoint_source(name: str, words, n: int, R: int, TB: int, lpos, seed: bool,
                    prefetch: bool = False, waves: int = None, readout=None, lanes=False,
-                   split_pairs: bool = False, lane_off: int = 0, interleave: bool = False) -> str:
+                   split_pairs: bool = False, lane_off: int = 0, interleave: bool = False,
+                   wave_sync: bool = False) -> str:
     """HIP source of one specialised adjoint pass kernel (the adjoint tile of
     qs_adjoint_pass with the pass program compiled in).  ``seed``: the first
     step of the sweep, whose co-state is built from psi and w = dL/dP, with
     the measured bit at position p the parity of ``readout[p]`` (None: bit p).
     ``lanes`` = "lds" and ``split_pairs`` as for pass_source, with the U^dagger
-    lane table (entries ``lane_off`` onward, adjoint_lane_entries order)."""
+    lane table (entries ``lane_off`` onward, adjoint_lane_entries order);
+    ``interleave`` and ``wave_sync`` as for pass_source."""
     K = R + TB
     T, NR = 1 << TB, 1 << R
     multi = n < K
@@ -808,7 +810,8 @@ def adjoint_source(name: str, words, n: int, R: int, TB: int, lpos, seed: bool,
     e(f"    const bool leader = (tid & {(1 << wlog) - 1}) == 0;")
     e(f"    float2 vp[{NR}], vl[{NR}];")
     for s, sub in enumerate(subs):
-        _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog, lane_k, interleave)
+        _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog, lane_k, interleave,
+                      wave_sync and s + 1 < len(subs) and wave_local(sub, subs[s + 1], TB))
     e("    const int tz = pinv(tid);")
     for it in range(NR // 2):
         e(f"    {{ const i64 gi = base | g_tid | {goff[it]}LL; const int sl = swz(2 * ({it * T} + tz)); const bool sw = sl & 1;")
@@ -859,7 +862,7 @@ def _emit_put(e, expr: str, row: int, wlog: int, indent: str):
     e(f"{indent}  if (leader) pinf(A.gpart)[(i64){row} * A.NS + uslot] = g; }}")
 
 
-def _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog, lanes=None, interleave=False):
+def _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog, lanes=None, interleave=False, wave_sync=False):
     NR = 1 << R
     e(f"    {{  // adjoint sub-pass {s}")
     e("      const int tq = pinv(tid);")
@@ -969,7 +972,11 @@ def _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog, lanes=None, interleave=False
             e("      }")
     for j in range(NR):
         e(f"      tp[lb ^ {swj[j]}] = vp[{phys[j]}]; tl[lb ^ {swj[j]}] = vl[{phys[j]}];")
-    e("      __syncthreads();")
+    if wave_sync:  # the next step keeps the wave-index bits (wave_local)
+        e("      __builtin_amdgcn_fence(__ATOMIC_RELEASE, \"wavefront\"); __builtin_amdgcn_wave_barrier();")
+        e("      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, \"wavefront\");")
+    else:
+        e("      __syncthreads();")
     e("    }")
 
 
@@ -1152,7 +1159,7 @@ class AdjointKernels:
 
     def __init__(self, passes, n: int, R: int, TB: int, seed_first: bool, prefetch: bool = False,
                  waves: int = 2, readout=None, grid: str = "full", lanes="lds", split_pairs: bool = True,
-                 interleave: bool = True):
+                 interleave: bool = True, wave_sync: bool = True):
         self.block = 1 << TB
         self.full_grid = grid == "full"
         self.sources, self.names = [], []
@@ -1164,7 +1171,7 @@ class AdjointKernels:
             if lanes == "lds":
                 self.lane_index += adjoint_lane_entries(words, R, TB)
             self.sources.append(adjoint_source(nm, words, n, R, TB, lpos, seed_first and i == 0, prefetch, waves,
-                                               readout, lanes, split_pairs, lane_off, interleave))
+                                               readout, lanes, split_pairs, lane_off, interleave, wave_sync))
             self.names.append(nm)
         self.funcs = kernels_for(self.sources, self.names)
         self.grid_cap = [None] * len(self.funcs)
