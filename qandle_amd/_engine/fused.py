@@ -261,7 +261,7 @@ def folds(prog: Program, specialise: bool) -> bool:
 # fusion budgets: the specialised kernels make per-gate work cheap, so passes
 # are packed to the planner's structural limit; the interpreting kernels pay
 # a dispatch per op and are kept lighter
-DEFAULT_COST = PL.CostModel(budget=1024.0, min_low=4)
+DEFAULT_COST = PL.CostModel(budget=1024.0, min_low=4, lookahead=True)
 INTERPRETED_COST = PL.CostModel(budget=256.0, min_low=4)
 
 

@@ -92,6 +92,7 @@ class CostModel:
     subpass: float = 3.0
     budget: float = 64.0        # VALU ops per amplitude per pass
     min_low: int = 5            # positions 0..min_low-1 are always local
+    lookahead: bool = False     # choose each pass's local positions by the score of the whole schedule
 
 
 def _gate_cost(g: Gate, in_regs: bool, cm: CostModel) -> float:
@@ -184,10 +185,75 @@ def plan(n: int, gates: typing.List[Gate], R: int, TB: int, cm: CostModel = None
     best = None
     for modes in ((True,), (False, True)):
         pl = _plan_passes(n, gates, R, TB, cm, modes)
-        score = 4 * len(pl.passes) + sum(len(p.subs) for p in pl.passes)
+        score = _score(pl)
         if best is None or score < best[0]:
             best = (score, pl)
+    if cm.lookahead and gates:
+        pl = _plan_lookahead(n, gates, R, TB, cm)
+        if _score(pl) < best[0]:
+            best = (_score(pl), pl)
     return best[1]
+
+
+def _score(pl: Plan) -> int:
+    return 4 * len(pl.passes) + sum(len(p.subs) for p in pl.passes)
+
+
+def _local_set(n: int, K: int, low, ranked):
+    L = list(low) + [p for p in ranked if p not in low][: K - len(low)]
+    for p in range(n):  # fill with the lowest unused positions
+        if len(L) == K:
+            break
+        if p not in L:
+            L.append(p)
+    return sorted(L)
+
+
+def _plan_lookahead(n: int, gates, R: int, TB: int, cm: CostModel) -> Plan:
+    """Each pass takes, among candidate local sets (the touch and need
+    rankings, and every window of K - min_low consecutive positions above the
+    always-local ones), the one whose pass plus the greedy schedule of what it
+    leaves scores lowest."""
+    K = R + TB
+    low = list(range(min(cm.min_low, K - 1)))
+    budget = cm.budget - cm.subpass * 2
+    high = [p for p in range(n) if p not in low]
+    windows = [high[i:i + K - len(low)] for i in range(0, max(len(high) - (K - len(low)), 0) + 1)]
+    pending = list(range(len(gates)))
+    passes = []
+    while pending:
+        best = None
+        cands = [_rank_positions(gates, pending, set(range(n)), t) for t in (True, False)] + windows
+        seen = set()
+        for ranked in cands:
+            L = _local_set(n, K, low, ranked)
+            if tuple(L) in seen:
+                continue
+            seen.add(tuple(L))
+            Lset = set(L)
+
+            def ok(g, Lset=Lset):
+                return all(p in Lset for p in g.need)
+
+            def cost(g, Lset=Lset):
+                return _gate_cost(g, g.kind != RZ or g.pos in Lset, cm)
+
+            done = _scan(gates, pending, ok, budget, cost)
+            if not done:
+                continue
+            subs = _pack(gates, done, L, R, cm)
+            ds = set(done)
+            rest = [gates[i] for i in pending if i not in ds]
+            score = 4 + len(subs) + (_score(_plan_passes(n, rest, R, TB, cm, (True,))) if rest else 0)
+            if best is None or score < best[0]:
+                best = (score, L, done, subs)
+        if best is None:
+            raise RuntimeError("planner made no progress")
+        _, L, done, subs = best
+        passes.append(Pass(L, subs, done))
+        ds = set(done)
+        pending = [i for i in pending if i not in ds]
+    return Plan(n, R, TB, passes)
 
 
 def _plan_passes(n: int, gates, R: int, TB: int, cm: CostModel, modes) -> Plan:

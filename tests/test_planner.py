@@ -204,3 +204,22 @@ def test_leading_rotations_go_into_the_product_factors(mode):
     p2 = fused.Program(3, g2, [fused.Slot("value")] * 2, ("input",), True)
     rest2, per2 = fused.leading_rotations(p2)
     assert [g.kind for g in rest2.gates] == [PL.CNOT, PL.RX] and list(per2) == [2]
+
+
+@pytest.mark.parametrize("mode", ["ladder", "random"])
+def test_lookahead_plan_matches_oracle_and_scores_no_worse(mode):
+    """The lookahead schedule (each pass's local set chosen by the score of the
+    whole remaining schedule) is a valid plan: emulated, it gives the oracle's
+    state; and it never scores worse than the greedy ones it is compared with."""
+    rng = np.random.default_rng(21 if mode == "ladder" else 22)
+    n, B, R, TB = 11, 2, 3, 4
+    ops = random_ops(rng, n, 4, B, mode)
+    gates, cs, efac = lower(n, ops, B)
+    greedy = PL.CostModel(budget=1024.0, min_low=2)
+    look = PL.CostModel(budget=1024.0, min_low=2, lookahead=True)
+    state, probs, pl = emulate_program(gates, n, B, R, TB, cs, efac, cm=look)
+    ref = fp64.simulate(n, ops)
+    np.testing.assert_allclose(state, ref, atol=1e-12)
+    np.testing.assert_allclose(probs, fp64.measure_prob(ref, n), atol=1e-12)
+    score = lambda p: 4 * len(p.passes) + sum(len(s.subs) for s in p.passes)  # noqa: E731
+    assert score(PL.plan(n, gates, R, TB, look)) <= score(PL.plan(n, gates, R, TB, greedy))
