@@ -207,7 +207,7 @@ def specialised_adjoint(prog, pl, ws, B, embed, seeded, fold=False):
 def run_with_grad(prog, inp, R, TB, cm=None):
     names = list(inp.named)
     vals = [inp.named[k] for k in names]
-    out = FusedCircuitFn.apply(prog, R, TB, cm or fused.DEFAULT_COST, inp.B, inp.batched, names, inp.phi_param,
+    out = FusedCircuitFn.apply(prog, R, TB, cm or fused.GRAD_COST, inp.B, inp.batched, names, inp.phi_param,
                                inp.embed_x, inp.state, *vals)
     if prog.measure:
         return None, out
