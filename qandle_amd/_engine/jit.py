@@ -103,6 +103,32 @@ __device__ __forceinline__ void mfma_pair2(float2 &x0, float2 &y0, float2 &x1, f
   x1 = make_float2(a1[0], a1[1]);
   y1 = make_float2(a1[2], a1[3]);
 }
+// four independent pairs, their MFMA chains interleaved: three other MFMAs
+// separate every dependent accumulation from its producer
+__device__ __forceinline__ void mfma_pair4(float2 &x0, float2 &y0, float2 &x1, float2 &y1, float2 &x2, float2 &y2,
+                                           float2 &x3, float2 &y3, const float (&m)[4]) {
+  qs_f4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = {0.f, 0.f, 0.f, 0.f}, a2 = {0.f, 0.f, 0.f, 0.f}, a3 = {0.f, 0.f, 0.f, 0.f};
+  a0 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[0], x0.x, a0, 0, 0, 0);
+  a1 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[0], x1.x, a1, 0, 0, 0);
+  a2 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[0], x2.x, a2, 0, 0, 0);
+  a3 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[0], x3.x, a3, 0, 0, 0);
+  a0 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[1], x0.y, a0, 0, 0, 0);
+  a1 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[1], x1.y, a1, 0, 0, 0);
+  a2 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[1], x2.y, a2, 0, 0, 0);
+  a3 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[1], x3.y, a3, 0, 0, 0);
+  a0 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[2], y0.x, a0, 0, 0, 0);
+  a1 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[2], y1.x, a1, 0, 0, 0);
+  a2 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[2], y2.x, a2, 0, 0, 0);
+  a3 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[2], y3.x, a3, 0, 0, 0);
+  a0 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[3], y0.y, a0, 0, 0, 0);
+  a1 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[3], y1.y, a1, 0, 0, 0);
+  a2 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[3], y2.y, a2, 0, 0, 0);
+  a3 = __builtin_amdgcn_mfma_f32_4x4x1f32(m[3], y3.y, a3, 0, 0, 0);
+  x0 = make_float2(a0[0], a0[1]); y0 = make_float2(a0[2], a0[3]);
+  x1 = make_float2(a1[0], a1[1]); y1 = make_float2(a1[2], a1[3]);
+  x2 = make_float2(a2[0], a2[1]); y2 = make_float2(a2[2], a2[3]);
+  x3 = make_float2(a3[0], a3[1]); y3 = make_float2(a3[2], a3[3]);
+}
 // coefficients of U^dagger (the transposed real 4x4 form): A_k = U[k][lane & 3]
 __device__ __forceinline__ void coeffs_dag(const int *up, float (&m)[4]) {
   cint *u = as_const(up);
@@ -207,7 +233,7 @@ def lane_entries(words, R: int, TB: int, multi: bool):
 def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, measure: bool,
                 prefetch: bool = False, waves: int = None, readout=None, valu_pairs: int = 0,
                 lanes=False, split_pairs: bool = False, lane_off: int = 0, interleave: bool = False,
-                wave_sync: bool = False) -> str:
+                wave_sync: bool = False, stagger: int = 0, stagger_cu: int = 256) -> str:
     """HIP source of one specialised forward pass kernel (extern "C" ``name``).
 
     ``prefetch`` keeps the next tile's HBM loads in flight in registers while
@@ -227,9 +253,12 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
     ``split_pairs``: tile loads and stores move each amplitude pair as two
     8-byte LDS accesses (the swizzle may swap a pair's halves) instead of one
     16-byte access with selects.  ``interleave``: the matrix-core updates of a
-    lane-table unitary run two amplitude pairs at a time with their MFMA
-    chains interleaved (mfma_pair2), which fills the dependent-accumulation
-    wait states; same arithmetic per pair, so results are bit for bit equal.
+    lane-table unitary run two (True or 2) or four (4) amplitude pairs at a
+    time with their MFMA chains interleaved (mfma_pair2, mfma_pair4), which
+    fills the dependent-accumulation wait states; same arithmetic per pair,
+    so results are bit for bit equal.  ``stagger``: the second workgroup of
+    each compute unit (workgroups ``stagger_cu`` to 2 ``stagger_cu`` - 1)
+    sleeps that many s_sleep 127 periods before starting (measured: no effect).
     ``wave_sync``: between two sub-passes that give the thread-index bits
     above the lane bits (the wave index) the same tile bits, each wave works
     on its own part of the tile and a wave-level fence replaces the block
@@ -255,6 +284,11 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
         lane_k = {ui: k for k, ui in enumerate(ents)}
         if ents:
             e(f"  __shared__ __align__(16) float4 ctab[{4 * len(ents)}];")
+    if stagger:
+        # the second workgroup of each compute unit starts later, so that the two alternate between
+        # streaming their tiles and computing on them instead of doing both in step
+        e(f"  if (blockIdx.x >= {stagger_cu} && blockIdx.x < {2 * stagger_cu}) "
+          f"for (int i = 0; i < {stagger}; ++i) __builtin_amdgcn_s_sleep(127);")
     e("  const int tid = threadIdx.x;")
     e(f"  const i64 total = A.B << {n};")
     e("  i64 g_tid = 0;")
@@ -524,15 +558,19 @@ def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu
                 nm = cq[(si_, r)]
                 e(f"        const float m[4] = {{{nm}.x, {nm}.y, {nm}.z, {nm}.w}};")
                 prs = [(phys[j], phys[j | (1 << r)]) for j in range(NR) if not (j >> r) & 1]
-                step = 2 if interleave else 1
-                for q in range(0, len(prs) - step + 1, step):
-                    if interleave:
-                        (a0, b0), (a1, b1) = prs[q], prs[q + 1]
+                step = 4 if interleave == 4 else (2 if interleave else 1)
+                q = 0
+                while q < len(prs):
+                    grp = prs[q:q + step]
+                    if len(grp) == 4:
+                        e("        mfma_pair4(" + ", ".join(f"v[{a}], v[{b}]" for a, b in grp) + ", m);")
+                    elif len(grp) >= 2:
+                        (a0, b0), (a1, b1) = grp[0], grp[1]
                         e(f"        mfma_pair2(v[{a0}], v[{b0}], v[{a1}], v[{b1}], m);")
+                        grp = grp[:2]
                     else:
-                        e(f"        mfma_pair(v[{prs[q][0]}], v[{prs[q][1]}], m);")
-                if interleave and len(prs) % 2:
-                    e(f"        mfma_pair(v[{prs[-1][0]}], v[{prs[-1][1]}], m);")
+                        e(f"        mfma_pair(v[{grp[0][0]}], v[{grp[0][1]}], m);")
+                    q += len(grp)
             elif ps and multi:
                 e("        const int *ut = pin(A.utab);")
                 e(f"        const int *u = ut + {off};")
@@ -1087,13 +1125,14 @@ class ProgramKernels:
     measuring pass drops to one (351 vs 256 us for that pass at 20 qubits).
     ``lanes`` = "lds" and ``split_pairs`` (pass_source) are the defaults: 3.48
     vs 3.67 ms per 20-qubit forward, 4.69 vs 4.89 at 24 qubits, results bit
-    for bit the same.  ``interleave`` (pass_source) is a default too: within
-    1% either way on these shapes, and it halves the MFMA wait states."""
+    for bit the same.  ``interleave`` (pass_source) = 4 is a default too:
+    within 1% of 2 and of none on these shapes, and each step halves the
+    MFMA wait states."""
 
     def __init__(self, passes, n: int, R: int, TB: int, gen_first: bool, measure: bool,
                  prefetch: bool = False, waves: int = 2, readout=None, valu_pairs: int = 0,
                  grid: str = "full", pipeline: int = 0, lanes="lds", split_pairs: bool = True,
-                 interleave: bool = True, wave_sync: bool = True):
+                 interleave: int = 4, wave_sync: bool = True, stagger: int = 0):
         self.n, self.R, self.TB = n, R, TB
         self.pipelined = bool(pipeline) and n >= R + TB
         self.block = (1 << TB) + ((1 << pipeline) if self.pipelined else 0)
@@ -1113,7 +1152,7 @@ class ProgramKernels:
             else:
                 self.sources.append(pass_source(nm, words, n, R, TB, lpos, gen_first and i == 0,
                                                 measure and i == npass - 1, prefetch, waves, readout, valu_pairs,
-                                                lanes, split_pairs, lane_off, interleave, wave_sync))
+                                                lanes, split_pairs, lane_off, interleave, wave_sync, stagger))
             self.names.append(nm)
         self.funcs = kernels_for(self.sources, self.names)
         self.grid_cap = [None] * len(self.funcs)
