@@ -151,6 +151,17 @@ def main():
     def sync():
         torch.cuda.synchronize(dev)
 
+    def settle(fn, seconds=0.3):
+        """Run fn (its first call compiles kernels) and then repeat it for ``seconds``: after a
+        compile the GPU idles for seconds, and the first timing that follows measured up to
+        1.5x slower than the same steps a moment later (25.7 vs 17.2 ms for the adjoint)."""
+        fn()
+        sync()
+        t = time.perf_counter()
+        while time.perf_counter() - t < seconds:
+            fn()
+            sync()
+
     def time_steps(fn, steps, warmup):
         for _ in range(warmup):
             fn()
@@ -179,14 +190,8 @@ def main():
             return circ(x=x)
 
     # first call (compiles the pass kernels), then ~0.3 s of the same steps before the W warm-up
-    # steps: the GPU's clocks rise over the first ~0.1 s of sustained work (the first of several
-    # timings in a process measured ~10% slower than the rest)
-    step()
-    sync()
-    t_settle = time.perf_counter()
-    while time.perf_counter() - t_settle < 0.3:
-        step()
-        sync()
+    # steps (the first of several timings in a process measured ~10% slower than the rest)
+    settle(step)
     gpu_s, wall_s = time_steps(step, args.steps, args.warmup)
     ms_per_step = gpu_s / args.steps * 1e3
     log(f"headline: {ms_per_step:.2f} ms per step")
@@ -284,6 +289,7 @@ def main():
         c24 = qandle.Circuit(w24.layers, num_qubits=24).to(dev)
         x24 = w24.inputs["x"].to(dev)
         with torch.no_grad():
+            settle(lambda: c24(x=x24))
             t24, _ = time_steps(lambda: c24(x=x24), 5, 2)
         p24 = fused.lower_circuit(c24.circuit)
         np24 = len(executed_plan(p24, *fused.choose_shape(24)).passes)
@@ -310,6 +316,7 @@ def main():
             x3.grad = None
             (c3.circuit.forward_layers(x=x3) * lw).sum().backward()
 
+        settle(adj_step)
         ta, _ = time_steps(adj_step, 6, 2)
         log(f"adjoint: {ta / 6 * 1e3:.2f} ms per forward + backward; plain per-gate autograd next")
         tp, _ = time_steps(plain_adj_step, 1, 1)
