@@ -22,7 +22,9 @@ AUTOTUNE = False         # time the kernel shape / fusion budget candidates on f
 AUTOTUNE_CACHE = None    # JSON file that keeps tuned choices across processes
 SPECIALISE = True        # passes run as kernels compiled at run time for the pass program (hipRTC);
                          # False runs the interpreting pass kernels of the library
-SPECIALISE_OPTIONS = {}  # forward kernel options: prefetch (bool), waves (int or None)
+SPECIALISE_OPTIONS = {}  # forward kernel options (jit.ProgramKernels): waves, lanes, split_pairs,
+                         # interleave, wave_sync, prefetch, grid, pipeline, stagger
 ABSORB_LEADING_ROTATIONS = True  # forward-only runs from a product state apply the rotations that precede
                                  # every CNOT on their qubit to that qubit's factor, not to the state
-SPECIALISE_ADJOINT_OPTIONS = {}  # adjoint sweep kernel options: prefetch (bool), waves (int or None)
+SPECIALISE_ADJOINT_OPTIONS = {}  # adjoint sweep kernel options (jit.AdjointKernels): waves, lanes,
+                                 # split_pairs, interleave, wave_sync, prefetch, grid
