@@ -297,6 +297,7 @@ def main():
         extras["n24_updates_per_s"] = w24.amplitude_updates() * 5 / t24
         extras["n24_ms_per_step"] = t24 / 5 * 1e3
         extras["n24_hbm_frac"] = b24 / (t24 / 5) / 1e9 / HBM_PEAK_GBS
+        extras["n24_per_gate_equivalent_hbm_frac"] = (4 * 2 ** 24 * (8 + 16 * len(w24.gates) + 8)) / (t24 / 5) / 1e9 / HBM_PEAK_GBS
         extras["n24_passes"] = np24
 
         # adjoint forward + backward, 16 qubits, batch 256, 20 layers
