@@ -73,7 +73,9 @@ def test_generated_adjoint_kernels_compile(n, R, TB, named, tmp_path, monkeypatc
 
 
 @pytest.mark.parametrize("opts", [{"lanes": "lds", "split_pairs": True}, {"lanes": True}, {"lanes": False,
-                                                                                         "split_pairs": False}])
+                                                                                         "split_pairs": False},
+                                  {"lanes": "lds", "split_pairs": True, "interleave": 4, "wave_sync": True,
+                                   "stagger": 2}])
 def test_pass_kernel_options_compile(opts, tmp_path, monkeypatch):
     """The operand-table and tile-move variants of the pass kernels (the engine
     default is lanes="lds" with split pairs) and the pipelined form compile."""
