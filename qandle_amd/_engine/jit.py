@@ -46,6 +46,11 @@ struct JArgs {
   i64 B; i64 ntiles; int store; int pad;
 };
 __device__ __forceinline__ int swz(int l) { int f = (l >> 5) ^ (l >> 10) ^ (l >> 15); return l ^ (f & 31); }
+// a tile element by byte offset: with the offset kept as (index << 3) the swizzled address of
+// register j is one XOR with a constant, (lb << 3) ^ (stride_j << 3)
+__device__ __forceinline__ float2 &lds_at(float2 *base, int byte_off) {
+  return *reinterpret_cast<float2 *>(reinterpret_cast<char *>(base) + byte_off);
+}
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -506,6 +511,8 @@ def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu
     e("      const bool live = sample < A.B;")
     e("      const i64 csample = live ? sample : 0;")
     e("      const int lb = swz(gl);")
+    if pipe is None:
+        e("      const int lb8 = lb << 3;")
     sw = [_swz(1 << sub.P[r]) for r in range(R)]
     swj = []
     for j in range(NR):
@@ -555,7 +562,10 @@ def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu
         e("      }")
     else:
         for j in range(NR):
-            e(f"      v[{j}] = tile[lb ^ {swj[j]}];")
+            if pipe is None:
+                e(f"      v[{j}] = lds_at(tile, lb8 ^ {8 * swj[j]});")
+            else:
+                e(f"      v[{j}] = tile[lb ^ {swj[j]}];")
     phys = list(range(NR))
     for si_, (bits, cnots) in enumerate(sub.segments):
         for r, ui, ps in bits:
@@ -625,7 +635,10 @@ def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu
             e(f"        v[{j}] = cmul(z, v[{j}]);")
         e("      }")
     for j in range(NR):
-        e(f"      tile[lb ^ {swj[j]}] = v[{phys[j]}];")
+        if pipe is None:
+            e(f"      lds_at(tile, lb8 ^ {8 * swj[j]}) = v[{phys[j]}];")
+        else:
+            e(f"      tile[lb ^ {swj[j]}] = v[{phys[j]}];")
     if measure:  # after the tile writes: the amplitudes' registers are free for the partials
         _measure(e, sub, phys, n, R, TB, multi, readout, pipe)
     if pipe is None and wave_sync:
@@ -922,6 +935,7 @@ def _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog, lanes=None, interleave=False
     e(f"      const i64 sample = {'gbase >> ' + str(n) if multi else 'block_sample'};")
     e("      const i64 csample = sample < A.B ? sample : 0;")
     e("      const int lb = swz(gl);")
+    e("      const int lb8 = lb << 3;")
     sw = [_swz(1 << sub.P[r]) for r in range(R)]
     swj = []
     for j in range(NR):
@@ -931,7 +945,7 @@ def _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog, lanes=None, interleave=False
                 a ^= sw[r]
         swj.append(a)
     for j in range(NR):
-        e(f"      vp[{j}] = tp[lb ^ {swj[j]}]; vl[{j}] = tl[lb ^ {swj[j]}];")
+        e(f"      vp[{j}] = lds_at(tp, lb8 ^ {8 * swj[j]}); vl[{j}] = lds_at(tl, lb8 ^ {8 * swj[j]});")
     cq = {}
     if lanes:
         for si_, (_cn, bits) in enumerate(sub.segments):
@@ -1036,7 +1050,7 @@ def _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog, lanes=None, interleave=False
                             e(f"        mfma_pair(vp[{a}], vp[{b}], cf); mfma_pair(vl[{a}], vl[{b}], cf);")
             e("      }")
     for j in range(NR):
-        e(f"      tp[lb ^ {swj[j]}] = vp[{phys[j]}]; tl[lb ^ {swj[j]}] = vl[{phys[j]}];")
+        e(f"      lds_at(tp, lb8 ^ {8 * swj[j]}) = vp[{phys[j]}]; lds_at(tl, lb8 ^ {8 * swj[j]}) = vl[{phys[j]}];")
     if wave_sync:  # the next step keeps the wave-index bits (wave_local)
         e("      __builtin_amdgcn_fence(__ATOMIC_RELEASE, \"wavefront\"); __builtin_amdgcn_wave_barrier();")
         e("      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, \"wavefront\");")
