@@ -194,3 +194,32 @@ def test_kernel_variants(device, R, TB):
     ref = fp64.simulate(w.n, w.oracle_ops())
     assert np.abs(psi.cpu().numpy() - ref).max() < ATOL
     assert np.abs(P.cpu().numpy() - fp64.measure_prob(ref, w.n)).max() < ATOL
+
+
+@pytest.mark.parametrize("n,B,seed", [(15, 3, 1), (17, 2, 2)])
+def test_random_long_range_cnots_vs_fp64(device, n, B, seed):
+    """Random rotations and long-range CNOTs (any control, any target) through the
+    compiled kernels and the lookahead planner, against the FP64 oracle."""
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator().manual_seed(seed)
+    x = torch.rand(B, n, generator=gen) * 3
+    layers = [qandle.AngleEmbedding(name="x", qubits=list(range(n)))]
+    ops = [("embed", "x", list(range(n)), x.double().numpy())]
+    for _ in range(6):
+        for q in range(n):
+            k = ("rx", "ry", "rz")[int(rng.integers(3))]
+            th = float(rng.uniform(0, 2 * np.pi))
+            layers.append({"rx": qandle.RX, "ry": qandle.RY, "rz": qandle.RZ}[k](q, theta=th, remapping=None))
+            ops.append((k, q, th))
+        for _ in range(n // 2):
+            c, t = (int(v) for v in rng.choice(n, 2, replace=False))
+            layers.append(qandle.CNOT(c, t))
+            ops.append(("cnot", c, t))
+    layers.append(qandle.MeasureProbability())
+    circ = qandle.Circuit(layers, num_qubits=n).to(device)
+    assert fused.lower_circuit(circ.circuit) is not None
+    with torch.no_grad():
+        P = circ(x=x.to(device)).cpu().numpy()
+    ref = fp64.measure_prob(fp64.simulate(n, ops), n)
+    print(f"random n={n}: max|P err| {np.abs(P - ref).max():.2e}")
+    assert np.abs(P - ref).max() < ATOL
