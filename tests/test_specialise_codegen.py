@@ -136,3 +136,29 @@ def test_lane_block_is_not_the_table_order_with_per_sample_unitaries():
     ws = fused.pass_words(prog, pl, B, "cpu")
     ents = [u for w, _ns, _lp in ws.passes for u in jit.lane_entries(w.numpy(), R, TB, n < R + TB)]
     assert ents and ents != list(range(len(ents)))
+
+
+def test_hermitian_trace_matches_the_full_trace():
+    """The adjoint kernels read Im Tr(Q M) for a generator Q = V P V^dagger
+    (Hermitian) from four sums over amplitude pairs (herm_trace):
+    Im M00, Im M11, Im M01 + Im M10 and Re M01 - Re M10, weighted by Q00,
+    Q11, Re Q10 and Im Q10.  Here the same algebra in numpy against Im Tr(Q M)
+    for random pairs, every Pauli generator and random unitaries V."""
+    rng = np.random.default_rng(5)
+    paulis = [np.array([[0, 1], [1, 0]], complex), np.array([[0, -1j], [1j, 0]]), np.diag([1.0, -1.0]).astype(complex)]
+    for _ in range(20):
+        z = rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2))
+        V, _ = np.linalg.qr(z)
+        psi = rng.normal(size=(8, 2)) + 1j * rng.normal(size=(8, 2))
+        lam = rng.normal(size=(8, 2)) + 1j * rng.normal(size=(8, 2))
+        M = sum(np.outer(p, np.conj(l)) for p, l in zip(psi, lam))
+        p0, p1, l0, l1 = psi[:, 0], psi[:, 1], lam[:, 0], lam[:, 1]
+        h0 = np.sum(p0.imag * l0.real - p0.real * l0.imag)
+        h1 = np.sum(p1.imag * l1.real - p1.real * l1.imag)
+        h2 = np.sum(p0.imag * l1.real - p0.real * l1.imag + p1.imag * l0.real - p1.real * l0.imag)
+        h3 = np.sum(p0.real * l1.real + p0.imag * l1.imag - p1.real * l0.real - p1.imag * l0.imag)
+        for P in paulis:
+            Q = V @ P @ V.conj().T
+            full = np.trace(Q @ M).imag
+            herm = Q[0, 0].real * h0 + Q[1, 1].real * h1 + Q[1, 0].real * h2 + Q[1, 0].imag * h3
+            assert abs(full - herm) < 1e-9 * max(1.0, abs(full))
