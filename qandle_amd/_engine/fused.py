@@ -12,6 +12,7 @@ from __future__ import annotations
 import dataclasses
 import math
 import typing
+import weakref
 
 import numpy as np
 import torch
@@ -147,13 +148,20 @@ def _lower_into(layers, n, mapping, prog_gates, slots, state):
     return True
 
 
+# built circuit -> its Program.  The program caches plans, device tables and
+# compiled-kernel handles (ctypes pointers), so it lives beside the module, not
+# on it: the module itself stays deep-copyable and picklable like any
+# nn.Module, and a copy lowers its own program (its slots then point at the
+# copy's parameters).
+_PROGRAMS: "weakref.WeakKeyDictionary" = weakref.WeakKeyDictionary()
+
+
 def lower_circuit(circuit) -> typing.Optional[Program]:
     """Program for a built circuit, or None when a layer is outside the program format."""
     from .. import qcircuit
 
-    cached = getattr(circuit, "_program", None)
-    if cached is not None:
-        return cached
+    if circuit in _PROGRAMS:
+        return _PROGRAMS[circuit]
     n = circuit.num_qubits
     gates: typing.List[PL.Gate] = []
     slots: typing.List[Slot] = []
@@ -175,10 +183,11 @@ def lower_circuit(circuit) -> typing.Optional[Program]:
         if ok and state.get("post"):
             ok = isinstance(layers[-1], (measurements.MeasureAllAbsolute, measurements.MeasureState))
     if not ok or n > 40 or n < 1:
+        _PROGRAMS[circuit] = None
         return None
     post = state.get("post")
     prog = Program(n, gates, slots, state["init"], bool(state.get("measured")), "abs2" if post == "abs2" else None)
-    circuit._program = prog
+    _PROGRAMS[circuit] = prog
     return prog
 
 

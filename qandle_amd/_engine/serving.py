@@ -148,10 +148,20 @@ class ServingEngine:
                 if self.inflight:
                     self.queue.insert(i, ch)
                     break
-                ch.req.error = "out of device memory at admission"
-                ch.req.chunks = []
-                ch.req.t_done = self.now()
-                self.done.append(ch.req)
+                self._reject(ch.req, "out of device memory at admission")
+                i = 0  # the queue lost the request's other chunks: scan it again
+
+    def _reject(self, req: Request, why: str):
+        """Fail a request once: its queued chunks leave the queue, chunks of it still
+        in flight are dropped when they retire (poll), and it is reported done now."""
+        if req.error is not None:
+            return
+        req.error = why
+        self.queue = collections.deque(c for c in self.queue if c.req is not req)
+        req.chunks = []
+        req.parts = {}
+        req.t_done = self.now()
+        self.done.append(req)
 
     def _launch(self, ch: _Chunk):
         req = ch.req
@@ -190,8 +200,10 @@ class ServingEngine:
                 retired += 1
                 self.resident -= ch.bytes
                 req = ch.req
-                req.parts[ch.lo] = ch.out
-                ch.out = ch.keep = None
+                out, ch.out, ch.keep = ch.out, None, None
+                if req.error is not None:
+                    continue  # the request was rejected while this chunk ran
+                req.parts[ch.lo] = out
                 req.chunks.remove((ch.lo, ch.hi))
                 if not req.chunks:
                     outs = [req.parts[k] for k in sorted(req.parts)]

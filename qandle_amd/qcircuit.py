@@ -132,7 +132,6 @@ class UnsplittedCircuit(torch.nn.Module):
         self.state = torch.zeros(2 ** num_qubits, dtype=torch.complex64)
         self.state[0] = 1
         self.layers = torch.nn.ModuleList(self._build_layers(layers, num_qubits))
-        self._program = None
 
     @staticmethod
     def _build_layers(layers: list, num_qubits: int) -> typing.List[torch.nn.Module]:
@@ -218,7 +217,6 @@ class SplittedCircuit(UnsplittedCircuit):
         self.subcircuits = torch.nn.ModuleList(
             [Subcircuit(num_qubits, subcircuitdict[i].mapping, subcircuitdict[i].layers) for i in subcircuitdict]
         )
-        self._program = None
 
     @staticmethod
     def _check_qubit_order(subcircuitdict):

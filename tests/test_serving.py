@@ -71,3 +71,65 @@ def test_results_identical_alone_and_in_mix(device):
         assert np.abs(reqs_mix[i].result.cpu().numpy() - ref).max() < 1e-5
     st = serving.latency_stats(done)
     assert st["p50_ms"] > 0 and st["p99_ms"] >= st["p50_ms"]
+
+
+class _DoneEvent:
+    def query(self):
+        return True
+
+
+def _host_engine(cap):
+    """A ServingEngine with its host-side state only (no streams, no device)."""
+    import collections
+    import time
+
+    eng = serving.ServingEngine.__new__(serving.ServingEngine)
+    eng.cap, eng.max_inflight, eng.max_bypass_s = int(cap), 64, 2.0
+    eng.queue, eng.inflight, eng.done = collections.deque(), [], []
+    eng.resident = eng.peak_resident = eng._rr = 0
+    eng.t0 = time.perf_counter()
+    return eng
+
+
+def test_oom_rejection_drops_every_chunk_of_the_request(monkeypatch):
+    """An allocator failure with nothing in flight rejects the whole request once:
+    its other queued chunks never launch and the loop keeps serving the rest."""
+    eng = _host_engine(3 * (2 ** 12) * 8 + 60_000)
+    big, _ = _req(0, 12, 10, 2, 1)
+    small, _ = _req(1, 10, 2, 2, 2)
+    eng.prepare(big)
+    eng.prepare(small)
+    assert len(big.chunks) >= 3 and len(small.chunks) == 1
+    launched = []
+
+    def fake_launch(ch):
+        if ch.req is big:
+            raise torch.cuda.OutOfMemoryError("fake allocator failure")
+        launched.append(ch)
+        ch.out, ch.event = torch.full((ch.hi - ch.lo, ch.req.n), 0.5), _DoneEvent()
+        eng.resident += ch.bytes
+        eng.inflight.append(ch)
+
+    monkeypatch.setattr(eng, "_launch", fake_launch)
+    eng.submit(big)
+    eng.submit(small)
+    while eng.inflight:
+        eng.poll()
+    assert [r.rid for r in eng.done].count(0) == 1 and [r.rid for r in eng.done].count(1) == 1
+    assert big.error is not None and big.result is None and big.chunks == []
+    assert not any(c.req is big for c in eng.queue) and not any(c.req is big for c in launched)
+    assert small.error is None and small.result.shape == (2, 10)
+    assert eng.resident == 0 and not eng.queue
+
+
+def test_chunk_of_a_rejected_request_is_dropped_when_it_retires():
+    eng = _host_engine(1e12)
+    r, _ = _req(0, 10, 4, 2, 3)
+    eng.prepare(r)
+    ch = serving._Chunk(r, 0, 4, 1000, event=_DoneEvent(), out=torch.zeros(4, 10))
+    eng.inflight.append(ch)
+    eng.resident = 1000
+    eng._reject(r, "rejected while running")
+    eng._reject(r, "a second failure is not reported twice")
+    eng.poll()
+    assert eng.done == [r] and r.result is None and eng.resident == 0 and not eng.inflight
