@@ -1,4 +1,12 @@
-"""StronglyEntanglingLayer (``src/qandle/ansaetze/stronglyentangling.py:17-133``)."""
+"""StronglyEntanglingLayer (``src/qandle/ansaetze/stronglyentangling.py:17-133``).
+
+The gate pattern comes from :func:`schedules.strongly_entangling`; this module
+only materialises it.  Public surface kept from the reference: the
+constructor arguments, the construction-time ``torch.rand(depth, len(qubits),
+len(rotations))`` draw of ``q_params``, ``StronglyEntanglingLayerBuilt.mods``
+(so ``state_dict`` keys are ``mods.<i>.theta``), ``decompose`` and the
+error types.
+"""
 from __future__ import annotations
 
 import copy
@@ -10,15 +18,22 @@ from .. import config
 from .. import operators as op
 from .. import qw_map
 from .. import utils
+from . import schedules
 from .ansatz import BuiltAnsatz, UnbuiltAnsatz
 
 __all__ = ["StronglyEntanglingLayer"]
 
 
+def _unbuilt(spec, q_params, remapping):
+    """Unbuilt operator of one schedule record (rotation angles from ``q_params``)."""
+    if isinstance(spec, schedules.CtrlSpec):
+        return spec.gate(spec.control, spec.target)
+    return spec.rot(qubit=spec.qubit, theta=q_params[spec.param], remapping=remapping)
+
+
 class StronglyEntanglingLayer(UnbuiltAnsatz):
-    """Per layer: the rotations (default RZ, RY, RZ) on every qubit of ``qubits``,
-    then a ring of CNOTs q_i -> q_(i + 1 + d mod (n_total - 1)) (``:17-78``).
-    ``q_params`` (depth, len(qubits), len(rotations)) defaults to ``torch.rand``."""
+    """Per layer: each of ``rotations`` (default RZ, RY, RZ) on every qubit of
+    ``qubits``, then a CNOT ring whose offset grows by one per layer."""
 
     def __init__(
         self,
@@ -30,73 +45,50 @@ class StronglyEntanglingLayer(UnbuiltAnsatz):
         remapping: typing.Union[typing.Callable, None] = config.DEFAULT_MAPPING,
     ):
         super().__init__()
-        self.depth = depth
-        self.qubits = qubits
-        self.num_qubits = num_qubits_total
+        self.qubits, self.num_qubits, self.depth = qubits, num_qubits_total, depth
         self.rots = [utils.parse_rot(r) for r in rotations]
-        if q_params is None:
-            q_params = torch.rand(depth, len(qubits), len(rotations))
-        self.q_params = q_params
-        if remapping is None:
-            remapping = qw_map.none
-        self.remapping = remapping
+        # drawn here, at construction (the built layer only reads it)
+        self.q_params = torch.rand(depth, len(qubits), len(rotations)) if q_params is None else q_params
+        self.remapping = qw_map.none if remapping is None else remapping
+
+    def _schedule(self, period, total):
+        return schedules.strongly_entangling(self.qubits, self.depth, self.rots, period, total, op.CNOT)
 
     def build(self, *args, **kwargs) -> BuiltAnsatz:
-        return StronglyEntanglingLayerBuilt(
-            num_qubits=kwargs["num_qubits"],
-            depth=self.depth,
-            rotations=self.rots,
-            q_params=self.q_params,
-            remapping=self.remapping,
-            qubits=self.qubits,
-        )
+        return StronglyEntanglingLayerBuilt(num_qubits=kwargs["num_qubits"], qubits=self.qubits, depth=self.depth,
+                                            rotations=self.rots, q_params=self.q_params, remapping=self.remapping)
 
     def __str__(self) -> str:
         return "SEL"
 
     def to_qasm(self):
-        raise NotImplementedError("TODO")  # the reference's unbuilt SEL has no export either (:57-58)
+        # the reference has no statement form for an unbuilt layer either (``:57-58``)
+        raise NotImplementedError("an unbuilt StronglyEntanglingLayer has no OpenQASM form; build it first")
 
-    def decompose(self) -> typing.List[op.UnbuiltOperator]:
-        layers = []
-        for d in range(self.depth):
-            for wi, w in enumerate(self.qubits):
-                for r in range(len(self.rots)):
-                    layers.append(self.rots[r](qubit=w, theta=self.q_params[d, wi, r], remapping=self.remapping))
-            layers.extend(StronglyEntanglingLayerBuilt._get_cnots(self.qubits, self.num_qubits, d % (self.num_qubits - 1)))
-        return layers
+    def decompose(self) -> typing.List[op.Operator]:
+        """Unbuilt rotations and built CNOTs, as the reference returns them
+        (``:60-75``); the ring offset here runs modulo ``num_qubits_total - 1``."""
+        total = self.num_qubits
+        out = []
+        for spec in self._schedule(total - 1, total):
+            g = _unbuilt(spec, self.q_params, self.remapping)
+            out.append(g.build(total) if isinstance(spec, schedules.CtrlSpec) else g)
+        return out
 
 
 class StronglyEntanglingLayerBuilt(BuiltAnsatz):
-    """Built SEL (``:80-133``): a ``torch.nn.Sequential`` of built rotations and
-    CNOTs (``mods``), so its ``state_dict`` keys are ``mods.<i>.theta``."""
+    """Built SEL (``:80-133``): its gates, built, in ``mods`` (a
+    ``torch.nn.Sequential``); the ring offset runs modulo ``len(qubits) - 1``."""
 
     def __init__(self, num_qubits: int, qubits: typing.List[int], depth: int, rotations, q_params: torch.Tensor,
                  remapping: typing.Union[typing.Callable, None]):
         super().__init__()
-        self.num_qubits = num_qubits
-        self.depth = depth
-        self.rots = rotations
-        self.qubits = qubits
-        layers = []
-        for d in range(depth):
-            for wi, w in enumerate(self.qubits):
-                for r in range(len(self.rots)):
-                    layers.append(self.rots[r](qubit=w, theta=q_params[d, wi, r], remapping=remapping).build(num_qubits))
-            layers.extend(self._get_cnots(self.qubits, self.num_qubits, d % (len(self.qubits) - 1)))
-        self.mods = torch.nn.Sequential(*layers)
+        self.num_qubits, self.qubits, self.depth, self.rots = num_qubits, qubits, depth, rotations
+        specs = schedules.strongly_entangling(qubits, depth, rotations, len(qubits) - 1, num_qubits, op.CNOT)
+        self.mods = torch.nn.Sequential(*[_unbuilt(s, q_params, remapping).build(num_qubits) for s in specs])
 
     def forward(self, state: torch.Tensor):
         return self.mods(state)
-
-    @staticmethod
-    def _get_cnots(qubits: typing.List[int], num_qubits_total: int, iteration: int) -> typing.List[op.BuiltCNOT]:
-        assert iteration + 1 < num_qubits_total
-        cnots = []
-        for ci in range(len(qubits)):
-            ti = (ci + iteration + 1) % len(qubits)
-            cnots.append(op.CNOT(qubits[ci], qubits[ti]).build(num_qubits_total))
-        return cnots
 
     def __str__(self) -> str:
         return "SEL"

@@ -14,6 +14,11 @@ DRAW_SHOW_VALUES = True        # False drops digits, dots, underscores, blanks a
 DRAW_SHIFT_LEFT = False        # True draws gates as far left as their wires allow
 DRAW_CROSS_BETWEEN_CNOT = True  # a crossing on the wires between a CNOT's control and target
 
+# Splitter: True carries a rotation's remapping into its sub-circuit, so split and unsplit circuits
+# compute the same angles; False rebuilds the gate with DEFAULT_MAPPING as the reference's splitter
+# does (src/qandle/splitter/main.py:143-153).
+SPLIT_KEEP_REMAPPING = True
+
 # Engine knobs (no counterpart in the reference).
 FUSED = True            # Circuit.forward uses the fused pass engine
 CHECKPOINT_LAYERS = 5   # adjoint backward restores psi from a checkpoint every k layers

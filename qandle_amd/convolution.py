@@ -1,18 +1,23 @@
 """QConv (``src/qandle/convolution.py:9-61``): a quantum convolution layer.
 
-Every output pixel is one small circuit: the unfolded input patch
-(channels x kernel, padded to 2^q) is amplitude-embedded, a
-StronglyEntanglingLayer runs on it and the joint basis-state probabilities
-are the output channels.  The patches of a batch are the batch of one
-circuit, so a layer call is one fused program over b x h_out x w_out samples
-(amplitude-embedding kernel, the SEL's passes, |psi|^2 readout).
+Each output pixel is one small circuit: the input patch under the kernel (all
+input channels, ``in_channels * kh * kw`` values, shifted by +0.001 so no
+patch is all zero) is amplitude-embedded into ``q`` qubits (zero-padded to
+2^q), a StronglyEntanglingLayer of depth ``qdepth`` acts on it and the joint
+basis-state probabilities are read out.  The first ``out_channels``
+probabilities, times ``out_channels / 2``, are the pixel's output channels.
+
+Engine mapping: the patches of the whole batch are the batch of ONE circuit
+call, so a layer forward is one fused program over ``b * h_out * w_out``
+samples (amplitude-embedding kernel, the SEL's passes, |psi|^2 readout)
+instead of one small circuit per pixel.  The host side here only gathers
+patches and scatters pixels, with plain tensor reshapes.
 """
 from __future__ import annotations
 
 import math
 import typing
 
-import einops
 import torch
 
 from . import embeddings, measurements, qcircuit
@@ -20,42 +25,53 @@ from .ansaetze import StronglyEntanglingLayer
 
 __all__ = ["QConv"]
 
+PATCH_SHIFT = 0.001  # added to every patch value before embedding (a zero patch has no direction)
+
+
+def _pair(v) -> typing.Tuple[int, int]:
+    return v if isinstance(v, tuple) else (v, v)
+
+
+def register_size(in_channels: int, out_channels: int, kernel: typing.Tuple[int, int]) -> int:
+    """Qubits of the patch circuit: enough amplitudes for a patch and enough
+    basis states for the output channels, at least one."""
+    need_in = math.ceil(math.log2(kernel[0] * kernel[1] * in_channels))
+    need_out = math.ceil(math.log2(out_channels))
+    return max(need_in, need_out, 1)
+
 
 class QConv(torch.nn.Module):
+    """Same constructor, attributes ``qubits``, ``unfold`` and ``qcircuit`` (so
+    ``state_dict`` keys ``qcircuit.circuit.layers.1.mods.<i>.theta``) and error
+    text as the reference."""
+
     def __init__(self, in_channels: int, out_channels: int,
                  kernel_size: typing.Union[typing.Tuple[int, int], int] = (3, 3),
                  padding: typing.Union[typing.Tuple[int, int], int] = (1, 1), qdepth: int = 1):
         super().__init__()
-        self.in_channels = in_channels
-        self.out_channels = out_channels
-        self.qdepth = qdepth
-        self.kernel_size = kernel_size if isinstance(kernel_size, tuple) else (kernel_size, kernel_size)
-        self.padding = padding if isinstance(padding, tuple) else (padding, padding)
+        self.in_channels, self.out_channels, self.qdepth = in_channels, out_channels, qdepth
+        self.kernel_size, self.padding = _pair(kernel_size), _pair(padding)
         self.unfold = torch.nn.Unfold(kernel_size=kernel_size, padding=self.padding)
-        qubits_for_inp = math.ceil(math.log2(self.kernel_size[0] * self.kernel_size[1] * in_channels))
-        qubits_for_out = math.ceil(math.log2(out_channels))
-        self.qubits = max(qubits_for_inp, qubits_for_out, 1)
-        amp = embeddings.AmplitudeEmbedding(qubits=list(range(self.qubits)), pad_with=0, name="emb")
-        sel = StronglyEntanglingLayer(qubits=list(range(self.qubits)), depth=self.qdepth)
-        mes = measurements.MeasureJointProbability()
-        self.qcircuit = qcircuit.Circuit(num_qubits=self.qubits, layers=[amp, sel, mes])
-        self.ein1 = "batch channel feat -> (batch feat) channel"
-        self.ein2 = "(batch h_out w_out) channel -> batch channel h_out w_out"
-        self.almost_zero = 0.001
+        self.qubits = register_size(in_channels, out_channels, self.kernel_size)
+        wires = list(range(self.qubits))
+        self.qcircuit = qcircuit.Circuit(
+            num_qubits=self.qubits,
+            layers=[embeddings.AmplitudeEmbedding(qubits=wires, pad_with=0, name="emb"),
+                    StronglyEntanglingLayer(qubits=wires, depth=qdepth),
+                    measurements.MeasureJointProbability()],
+        )
 
-    def _post_process(self, x):
-        x = x[:, : self.out_channels, :, :]  # remove padding
-        return x * self.out_channels / 2  # rescale
+    def output_size(self, h: int, w: int) -> typing.Tuple[int, int]:
+        (kh, kw), (ph, pw) = self.kernel_size, self.padding
+        return h + 2 * ph - kh + 1, w + 2 * pw - kw + 1
 
     def forward(self, x):
-        b, c_in, h_in, w_in = x.shape
-        h_out = h_in + 2 * self.padding[0] - self.kernel_size[0] + 1
-        w_out = w_in + 2 * self.padding[1] - self.kernel_size[1] + 1
+        b, c_in, h, w = x.shape
         if c_in != self.in_channels:
             raise ValueError(f"Input channels {c_in} does not match in_channels {self.in_channels}")
-        x = self.unfold(x)
-        x = einops.rearrange(x, self.ein1)
-        x = x + self.almost_zero  # avoid zero input
-        x = self.qcircuit(emb=x)
-        x = einops.rearrange(x, self.ein2, batch=b, h_out=h_out, w_out=w_out)
-        return self._post_process(x)
+        h_out, w_out = self.output_size(h, w)
+        cols = self.unfold(x)                                  # (b, c_in*kh*kw, h_out*w_out)
+        patches = cols.transpose(1, 2).reshape(-1, cols.shape[1]) + PATCH_SHIFT  # pixel-major rows
+        probs = self.qcircuit(emb=patches)                     # (b*h_out*w_out, 2^q)
+        img = probs.reshape(b, h_out, w_out, -1).permute(0, 3, 1, 2)
+        return img[:, : self.out_channels] * (self.out_channels / 2)

@@ -268,6 +268,9 @@ class Subcircuit(torch.nn.Module):
                 if phi.dim() == 1 and st.shape[0] == 1 and phi.shape[0] != 1:
                     st = st.expand(phi.shape[0], -1)
                 st = plain.RotFn.apply(st, phi, n, n - 1 - self.inverse[mod.qubit], mod.KIND)
+            elif isinstance(mod, operators.BuiltU):
+                parent_bit = n - 1 - self.inverse[mod.qubit]
+                st = plain.UnitaryFn.apply(st, mod._u.to(st.device), n, parent_bit, 0)
             elif isinstance(mod, measurements.MeasurementDummy):
                 st = mod(st)
             else:

@@ -7,10 +7,14 @@ group of the nearest CNOT on its qubit (``main.py:58-105``) and renumbering of
 each group's qubits (``main.py:114-157``).  The split only changes how the
 reference executes (smaller dense matrices); this engine lowers the
 sub-circuits back onto parent bit positions, so a split circuit computes the
-same result as the unsplit one.  One deliberate difference: a rotation's
-``remapping`` is carried into its sub-circuit (the reference rebuilds the gate
-with the default mapping, ``main.py:148-153``, which makes split and unsplit
-results differ for ``remapping=None``).
+same result as the unsplit one.
+
+One deliberate difference, switchable: a rotation's ``remapping`` is carried
+into its sub-circuit (``config.SPLIT_KEEP_REMAPPING``, default True).  The
+reference rebuilds every rotation from its class, theta and name only
+(``main.py:143-153``), so the gate falls back to ``config.DEFAULT_MAPPING``
+and a split circuit of ``remapping=None`` gates computes other angles than the
+unsplit one; ``SPLIT_KEEP_REMAPPING = False`` reproduces that rebuild.
 """
 from __future__ import annotations
 
@@ -111,10 +115,18 @@ def _normalize(gates) -> SubcircuitContainer:
         else:
             qubits.add(g.qubit)
     mapping = {q: i for i, q in enumerate(sorted(qubits))}
+    from .. import config
+
+    keep_remap = getattr(config, "SPLIT_KEEP_REMAPPING", True)
     new_layers = []
     for g in gates:
         if _is_cnot(g):
             new_layers.append(op.CNOT(control=mapping[g.c], target=mapping[g.t]))
+            continue
+        if isinstance(g, op.CustomGate):
+            # a custom gate is built already: rebuilt for the sub-register (main.py:131-138)
+            new_layers.append(op.CustomGate(qubit=mapping[g.qubit], matrix=g.original_matrix,
+                                            num_qubits=len(mapping), self_description=g.description))
             continue
         cls = op.BUILT_CLASS_RELATION.T[type(g)] if isinstance(g, op.BuiltOperator) else type(g)
         kwargs = {}
@@ -122,7 +134,7 @@ def _normalize(gates) -> SubcircuitContainer:
             kwargs["theta"] = g.theta
         if getattr(g, "name", None) is not None:
             kwargs["name"] = g.name
-        if hasattr(g, "remapping"):
+        if keep_remap and hasattr(g, "remapping"):
             kwargs["remapping"] = g.remapping
         new_layers.append(cls(qubit=mapping[g.qubit], **kwargs))
     return SubcircuitContainer(new_layers, mapping)

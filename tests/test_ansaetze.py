@@ -114,3 +114,68 @@ def test_lowering():
         assert fused.lower_circuit(sb.circuit) is not None
     for other in (qandle.TwoLocal(depth=2), qandle.SU(reps=1)):
         assert fused.lower_circuit(qandle.Circuit([other], num_qubits=n).circuit) is None
+
+
+def _layout(sb):
+    out = []
+    for g in sb.layers_ub:
+        if hasattr(g, "qubit"):
+            out.append((type(g).__name__, g.qubit))
+        else:
+            out.append((type(g).__name__, g.c, g.t))
+    return out
+
+
+def test_budget_layout_edge_cases():
+    """Walks of the generator at src/qandle/ansaetze/stronglyentangling_budget.py:50-75,
+    done by hand: a budget that fills whole layers still opens the next layer
+    (its ring is kept, it places no rotation); spacing 2 skips every other ring;
+    the ring offset is (k mod Q) mod (Q - 1) for the k-th ring."""
+    rz, ry, cx = "RZ", "RY", "CNOT"
+    # 3 qubits, budget 6, a ring before every layer
+    got = _layout(qandle.StronglyEntanglingLayerBudget(3, param_budget=6))
+    assert got == [(rz, 0), (rz, 1), (rz, 2), (cx, 0, 1), (cx, 1, 2), (cx, 2, 0),
+                   (ry, 0), (ry, 1), (ry, 2), (cx, 0, 2), (cx, 1, 0), (cx, 2, 1)]
+    # 3 qubits, budget 7, a ring before every second layer
+    got = _layout(qandle.StronglyEntanglingLayerBudget(3, param_budget=7, control_gate_spacing=2))
+    assert got == [(rz, 0), (rz, 1), (rz, 2), (ry, 0), (ry, 1), (ry, 2),
+                   (cx, 0, 1), (cx, 1, 2), (cx, 2, 0), (rz, 0)]
+    # 4 qubits, six rings: offsets 0, 1, 2, 0, 0, 1
+    sb = qandle.StronglyEntanglingLayerBudget(4, param_budget=24, rotations=["ry"])
+    rings = [g for g in sb.layers_ub if not hasattr(g, "qubit")]
+    offsets = [(g.t - g.c - 1) % 4 for g in rings[0::4]]
+    assert offsets == [0, 1, 2, 0, 0, 1]
+    assert all((rings[4 * k + i].c, rings[4 * k + i].t) == (i, (i + 1 + offsets[k]) % 4)
+               for k in range(6) for i in range(4))
+    # no budget, no gates; one qubit cannot form a ring
+    assert qandle.StronglyEntanglingLayerBudget(3, param_budget=0).layers_ub == []
+    with pytest.raises(ZeroDivisionError):
+        qandle.StronglyEntanglingLayerBudget(1, param_budget=2)
+    with pytest.raises(AssertionError):
+        qandle.StronglyEntanglingLayerBudget(3, param_budget=2, control_gate=qandle.SWAP)
+
+
+def test_sel_decompositions():
+    """Built and unbuilt SEL list the same gates when the layer spans the register;
+    on a sub-register the unbuilt form's ring offset runs modulo the register size
+    minus one (src/qandle/ansaetze/stronglyentangling.py:75 vs :104)."""
+    q = torch.rand(4, 3, 3)
+    full = qandle.StronglyEntanglingLayer(qubits=[0, 1, 2], num_qubits_total=3, depth=4, q_params=q)
+    def sig(gates):
+        return [(type(g).__name__.replace("Built", ""), getattr(g, "qubit", None), getattr(g, "c", None),
+                 getattr(g, "t", None)) for g in gates]
+
+    unb = full.decompose()
+    assert all(isinstance(g, qandle.operators.BuiltCNOT) for g in unb if hasattr(g, "c"))  # as the reference
+    assert sig(unb) == sig(full.build(num_qubits=3).decompose())
+    sub = qandle.StronglyEntanglingLayer(qubits=[1, 2, 3], num_qubits_total=5, depth=3, q_params=q[:3])
+    built = [g for g in sub.build(num_qubits=5).mods if isinstance(g, qandle.operators.BuiltCNOT)]
+    assert [(g.c, g.t) for g in built[6:9]] == [(1, 2), (2, 3), (3, 1)]    # offset 2 % 2 = 0
+    # unbuilt: offset 2 % 4 = 2 turns the 3-wire ring onto itself, and CNOT(q, q) is refused
+    with pytest.raises(AssertionError):
+        sub.decompose()
+    sub2 = qandle.StronglyEntanglingLayer(qubits=[1, 2, 3], num_qubits_total=5, depth=2, q_params=q[:2])
+    rings = [g for g in sub2.decompose() if isinstance(g, qandle.operators.BuiltCNOT)]
+    assert [(g.c, g.t) for g in rings] == [(1, 2), (2, 3), (3, 1), (1, 3), (2, 1), (3, 2)]  # offsets 0, 1
+    with pytest.raises(TypeError):
+        qandle.StronglyEntanglingLayer(qubits=[0, 1], depth=1).decompose()  # no register size given
