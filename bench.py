@@ -8,19 +8,31 @@ one forward of that circuit over the batch, inputs already resident in HBM.
 Updates per step = sum over gate applications of B * 2^n (each embedded qubit
 counts as one gate).
 
-Also reported: the plain per-gate GPU path on the same workload, the fused
-forward at 24 qubits (batch 4), the adjoint forward + backward at 16 qubits
-(batch 256, 20 layers) against the plain per-gate path, the roofline of the
-dominant kernel (the fused pass kernel), and the CPU restatement of qandle's
-complex64 torch path timed on a bounded sample on this host.
+``roofline`` is the HBM roofline of the dominant kernel, the fused pass kernels
+``qs_pass_<i>``: algorithmic bytes per launch (first pass 8 B per amplitude,
+middle passes 16 B, the measuring last pass 8 B, over the executed schedule)
+divided by the launches' average duration from HIP events on their own
+stream, against 8 TB/s.  ``roofline.mfma`` is the same launches against the
+fp32 matrix-core peak (16 flops per amplitude per fused 2x2 unitary).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+``extras``: the plain per-gate GPU path, 24 qubits x 4, the adjoint forward +
+backward at 16 qubits x 256 x 20 layers (with the roofline of its ``qs_adj_<i>``
+kernels) against plain per-gate autograd, the 2048-circuit uneven batch (with
+the roofline of its level launches), the serving loop, and the 8-qubit CPU
+configuration (the reference's dense cached-matrix path restated on the host,
+beside the same circuit on the GPU).  ``cpu_baseline``: the structured
+complex64 restatement of qandle's gate semantics on the host's CPU share.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick] [--no-cpu]
+    python bench.py --profile fused|adjoint|uneven|copy   (short runs for rocprofv3)
+    python bench.py --summarize RESULTS_DB [--last N]    (steady-state kernel table)
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -29,7 +41,6 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 MFMA_F32_PEAK_TFLOPS = 157.3  # dense fp32 matrix-core peak of one MI355X
-CPU_THREADS = 16
 _T0 = time.perf_counter()
 
 
@@ -45,42 +56,131 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--quick", action="store_true", help="headline only (no extra workloads, no CPU baseline)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile", choices=["fused", "adjoint", "copy"],
+    ap.add_argument("--profile", choices=["fused", "adjoint", "uneven", "copy"],
                     help="run only the named workload a few times (for rocprofv3 counter and trace runs)")
+    ap.add_argument("--summarize", metavar="DB", help="print the kernel table of a rocprofv3 results database")
+    ap.add_argument("--last", type=int, default=0,
+                    help="with --summarize: only the last N dispatches of every kernel (the steady state)")
     return ap.parse_args()
 
 
-def profile_workload(mode, reps=2):
-    """Short fixed runs for profilers: the headline forward, its forward + adjoint
-    backward, or a device copy of the same state size (the HBM counter calibration)."""
+# ----------------------------------------------------------------------------
+# workloads
+# ----------------------------------------------------------------------------
+
+def headline_workload():
+    from qandle_amd import workloads
+
+    return workloads.rxryrz_ladder(n=20, B=64, depth=10, seed=1)
+
+
+def adjoint_workload():
+    from qandle_amd import workloads
+
+    return workloads.rxryrz_ladder(n=16, B=256, depth=20, seed=2, loss=True)
+
+
+def uneven_workloads():
+    import torch
+
+    from qandle_amd import workloads
+
+    ws = workloads.random_uneven(2048, build_layers=False)
+    gen = torch.Generator().manual_seed(3)
+    return ws, [torch.randn(w.B, w.n, generator=gen) for w in ws]
+
+
+def settle(fn, sync, seconds=0.3):
+    """Run fn (its first call compiles kernels), then repeat it for ``seconds``: after a
+    compile the GPU idles for seconds, and the first timing that followed measured up to
+    1.5x slower than the same steps a moment later."""
+    fn()
+    sync()
+    t = time.perf_counter()
+    while time.perf_counter() - t < seconds:
+        fn()
+        sync()
+
+
+def profile_workload(mode, reps=4):
+    """Fixed runs for profilers: ``reps`` steady-state steps after the first call and a
+    settling period (summarize with --last reps x kernels-per-step)."""
     import torch
 
     import qandle_amd as qandle
-    from qandle_amd import workloads
 
     dev = torch.device("cuda:0")
-    n, B = 20, 64
+
+    def sync():
+        torch.cuda.synchronize(dev)
+
     if mode == "copy":
+        n, B = 20, 64
         a = torch.empty(B * 2 ** n, dtype=torch.complex64, device=dev)
         b = torch.empty_like(a)
         for _ in range(reps + 1):
             b.copy_(a)
-        torch.cuda.synchronize(dev)
+        sync()
         print(f"copy: {a.numel() * 8} bytes read and written per launch", flush=True)
         return
-    w = workloads.rxryrz_ladder(n=n, B=B, depth=10, seed=1)
-    circ = qandle.Circuit(w.layers, num_qubits=n).to(dev)
+    if mode == "uneven":
+        from qandle_amd._engine import fused
+        from qandle_amd._engine.uneven import UnevenBatch
+
+        ws, lw = uneven_workloads()
+        ub = UnevenBatch.from_workloads(ws, dev, R=4, TB=8, cost=fused.PL.CostModel(budget=160.0, min_low=4))
+        ub.set_loss_weights(lw)
+        settle(ub.run_forward, sync)
+        for _ in range(reps):
+            ub.run_forward()
+        sync()
+        print(f"uneven: {reps} forwards of 2048 circuits", flush=True)
+        return
+    w = headline_workload() if mode == "fused" else adjoint_workload()
+    circ = qandle.Circuit(w.layers, num_qubits=w.n).to(dev)
     x = w.inputs["x"].to(dev)
-    for _ in range(reps):
+
+    def step():
         if mode == "fused":
             with torch.no_grad():
                 circ(x=x)
         else:
             xg = x.clone().requires_grad_(True)
-            circ(x=xg).sum().backward()
-    torch.cuda.synchronize(dev)
-    print(f"{mode}: {reps} runs of {w.name}", flush=True)
+            (circ(x=xg) * w.loss_w.to(dev)).sum().backward()
 
+    settle(step, sync)
+    for _ in range(reps):
+        step()
+    sync()
+    print(f"{mode}: {reps} steady-state steps of {w.name}", flush=True)
+
+
+def summarize(db: str, last: int):
+    """Kernel table (calls, total, average, min, max in microseconds) of a rocprofv3
+    results database; with ``last`` > 0 only the last ``last`` dispatches of every
+    kernel name count (the steady-state steps after compiles and settling)."""
+    import sqlite3
+
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, start, end from kernels order by start").fetchall()
+    by = {}
+    for name, s, e in rows:
+        by.setdefault(name, []).append((e - s) / 1e3)
+    print(f"| Kernel | Calls | Total (us) | Average (us) | Min (us) | Max (us) |")
+    print("|---|---|---|---|---|---|")
+    out = []
+    for name, ds in by.items():
+        if last > 0:
+            ds = ds[-last:]
+        out.append((sum(ds), name, ds))
+    for tot, name, ds in sorted(out, reverse=True):
+        short = name if len(name) < 70 else name[:67] + "..."
+        print(f"| `{short}` | {len(ds)} | {tot:.1f} | {tot / len(ds):.1f} | {min(ds):.1f} | {max(ds):.1f} |")
+
+
+# ----------------------------------------------------------------------------
+# helpers for the headline roofline
+# ----------------------------------------------------------------------------
 
 def executed_program(prog):
     """The program a forward without gradients runs from a product state: the
@@ -103,33 +203,80 @@ def executed_plan(prog, R, TB):
     return fused.get_plan(p, R, TB, None, fused.folds(p, fused.use_jit(p.n, R, TB)))
 
 
-def fused_unitaries(prog, plan, B):
-    """Fused 2x2 unitaries the executed schedule applies to every amplitude, all passes together."""
+def pass_structure(prog, plan, B):
+    """(fused 2x2 unitaries, sub-passes) of every pass of the executed schedule."""
     from qandle_amd._engine import fused
     from qandle_amd._engine import planner as PL
 
     ws = fused.pass_words(executed_program(prog), plan, B, "cpu")
-    return sum(len(bits) for w, _ns, _lp in ws.passes for sub in PL.decode_pass(w.numpy(), plan.R, plan.TB)
-               for bits, _cn in sub.segments)
+    out = []
+    for w, nsub, _lp in ws.passes:
+        subs = PL.decode_pass(w.numpy(), plan.R, plan.TB)
+        out.append((sum(len(bits) for sub in subs for bits, _cn in sub.segments), nsub))
+    return out
 
 
-def fused_bytes_per_launch(B, n, npass, stores_last):
-    """Algorithmic HBM bytes of the executed schedule, averaged per pass launch.
-
-    First pass: generates the embedding product state in registers and writes
-    it (8 B/amplitude).  Middle passes: read + write (16 B).  Last pass: read
-    (8 B), plus a write when the state is kept.  Measurement partials are
-    K + 1 (or, with a folded readout, n + 1) floats per tile, negligible.
-    """
+def pass_bytes(B, n, npass, stores_last):
+    """Algorithmic HBM bytes of every pass launch: the first pass generates the
+    embedding product state in registers and writes it (8 B per amplitude), middle
+    passes read and write (16 B), the last pass reads (8 B, plus a write when the
+    state is kept).  Measurement partials are K + 1 (or n + 1) floats per tile."""
     amps = B * 2 ** n
-    per_amp = 8 + 16 * max(npass - 2, 0) + (8 + (8 if stores_last else 0) if npass > 1 else 0)
     if npass == 1:
-        per_amp = 8 if stores_last else 0
-    return amps * per_amp / npass, amps * per_amp
+        return [amps * (8 if stores_last else 0)]
+    return [amps * 8] + [amps * 16] * (npass - 2) + [amps * (16 if stores_last else 8)]
 
+
+def hbm_roofline(events, kernel):
+    """HBM roofline dict from (start, end, bytes) launch records."""
+    durs = [a.elapsed_time(b) / 1e3 for a, b, _ in events]
+    nbytes = sum(x for _, _, x in events)
+    achieved = nbytes / sum(durs) / 1e9
+    return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "avg_launch_us": sum(durs) / len(durs) * 1e6,
+            "alg_bytes_per_launch": nbytes / len(durs), "launches_timed": len(durs)}
+
+
+def cpu_share() -> int:
+    """Host CPUs this process may use: its affinity set, capped by OMP_NUM_THREADS
+    when the environment sets it (the box's per-GPU CPU share)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(n, 1)
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def median_time(fn, runs=5):
+    fn()  # warm-up
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return statistics.median(ts), ts
+
+
+# ----------------------------------------------------------------------------
+# main
+# ----------------------------------------------------------------------------
 
 def main():
     args = parse()
+    if args.summarize:
+        summarize(args.summarize, args.last)
+        return
     if args.profile:
         profile_workload(args.profile)
         return
@@ -141,7 +288,6 @@ def main():
     import torch
 
     import qandle_amd as qandle
-    from qandle_amd import workloads
     from qandle_amd._engine import fused, lib
 
     lib.load()
@@ -150,17 +296,6 @@ def main():
 
     def sync():
         torch.cuda.synchronize(dev)
-
-    def settle(fn, seconds=0.3):
-        """Run fn (its first call compiles kernels) and then repeat it for ``seconds``: after a
-        compile the GPU idles for seconds, and the first timing that follows measured up to
-        1.5x slower than the same steps a moment later (25.7 vs 17.2 ms for the adjoint)."""
-        fn()
-        sync()
-        t = time.perf_counter()
-        while time.perf_counter() - t < seconds:
-            fn()
-            sync()
 
     def time_steps(fn, steps, warmup):
         for _ in range(warmup):
@@ -178,8 +313,8 @@ def main():
 
     # ---------------------------------------------------------------- headline
     log("headline: 20 qubits, batch 64, 10 layers (first call compiles the pass kernels)")
-    n, B, depth = 20, 64, 10
-    w = workloads.rxryrz_ladder(n=n, B=B, depth=depth, seed=1)
+    w = headline_workload()
+    n, B, depth = w.n, w.B, 10
     circ = qandle.Circuit(w.layers, num_qubits=n).to(dev)
     x = w.inputs["x"].to(dev)
     prog = fused.lower_circuit(circ.circuit)
@@ -189,40 +324,49 @@ def main():
         with torch.no_grad():
             return circ(x=x)
 
-    # first call (compiles the pass kernels), then ~0.3 s of the same steps before the W warm-up
-    # steps (the first of several timings in a process measured ~10% slower than the rest)
-    settle(step)
+    settle(step, sync)
     gpu_s, wall_s = time_steps(step, args.steps, args.warmup)
     ms_per_step = gpu_s / args.steps * 1e3
-    log(f"headline: {ms_per_step:.2f} ms per step")
+    log(f"headline: {ms_per_step:.3f} ms per step")
     updates = w.amplitude_updates()
     value = updates * args.steps / gpu_s
 
-    # per-launch durations of the dominant kernel, on its own stream, over K steps
+    # per-launch durations of the dominant kernels, on their own stream
     plan = executed_plan(prog, R, TB)
     npass = len(plan.passes)
+    pbytes = pass_bytes(B, n, npass, stores_last=False)
+    struct = pass_structure(prog, plan, B)
     with torch.no_grad():
         inp = fused.gather_inputs(prog, circ.circuit, None, {"x": x})
         cs = fused.build_coeffs(prog, inp)
         efac = fused.product_factors(prog, inp)
-        events = []
-        for _ in range(max(2, args.steps // 4)):
-            fused.execute(prog, inp, cs, efac, R, TB, want_state=False, events=events)
+        raw = []
+        reps = max(2, args.steps // 4)
+        for _ in range(reps):
+            fused.execute(prog, inp, cs, efac, R, TB, want_state=False, events=raw)
         sync()
-    durs = [a.elapsed_time(b) / 1e3 for a, b in events]
-    avg_launch = sum(durs) / len(durs)
-    bpl, bstep = fused_bytes_per_launch(B, n, npass, stores_last=False)
-    achieved = bpl / avg_launch / 1e9
-    # matrix-core work: each fused unitary is a 2x2 complex matvec per amplitude pair,
-    # 16 real multiply-adds = 32 flops per pair, 16 per amplitude
-    units = fused_unitaries(prog, plan, B)
-    fpl = units / npass * B * 2 ** n * 16
-    tflops = fpl / avg_launch / 1e12
+    events = [(a, b, pbytes[i % npass]) for i, (a, b) in enumerate(raw)]
+    kernel = (f"qs_pass_<i> (pass kernels specialised per pass program, hipRTC; R={R}, TB={TB})"
+              if fused.use_jit(n, R, TB) else f"k_pass<{R},{TB},false> (qs_fused_pass)")
+    roof = hbm_roofline(events, kernel)
+    per_pass = []
+    for i in range(npass):
+        ds = [a.elapsed_time(b) * 1e3 for a, b, _ in events[i::npass]]
+        per_pass.append({"us": sum(ds) / len(ds), "unitaries": struct[i][0], "sub_passes": struct[i][1],
+                         "alg_bytes": pbytes[i]})
+    units = sum(u for u, _ in struct)
+    flops_per_launch = units / npass * B * 2 ** n * 16
+    tflops = flops_per_launch / (roof["avg_launch_us"] / 1e6) / 1e12
+    roof["mfma"] = {"achieved": tflops, "peak": MFMA_F32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": tflops / MFMA_F32_PEAK_TFLOPS, "alg_flops_per_launch": flops_per_launch,
+                    "fused_unitaries_per_step": units}
+    roof["per_pass"] = per_pass
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_fused_pass_n20_b64.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+    roof["traffic"] = traffic
 
     result = {
         "metric": "amplitude-gate updates/s, % HBM peak, fused vs plain; p50/p99 serve latency",
@@ -243,100 +387,42 @@ def main():
             "updates_per_step": updates, "fused_passes": npass, "tile": {"R": R, "TB": TB},
             "parallelism": "single GPU",
         },
-        "roofline": {
-            # the pass kernels carry ~26 fused unitaries each: the fp32 matrix-core rate is the
-            # nearer roof (the HBM figures of the same launches are kept beside it)
-            "bound": "mfma",
-            "kernel": (f"qs_pass_<i> (pass kernels specialised per pass program, hipRTC; R={R}, TB={TB})"
-                       if fused.use_jit(n, R, TB) else f"k_pass<{R},{TB},false> (qs_fused_pass)"),
-            "achieved": tflops,
-            "peak": MFMA_F32_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": tflops / MFMA_F32_PEAK_TFLOPS,
-            "traffic": traffic,
-            "avg_launch_us": avg_launch * 1e6,
-            "alg_flops_per_launch": fpl,
-            "fused_unitaries_per_step": units,
-            "hbm": {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                    "alg_bytes_per_launch": bpl},
-            "launches_timed": len(durs),
-        },
+        "roofline": roof,
         "wall_s_timed_region": wall_s,
     }
 
     if not args.quick:
         extras = {}
-        # plain per-gate GPU path on the same workload
         log("plain per-gate path on the headline workload")
 
         def plain_step():
             with torch.no_grad():
                 return circ.circuit.forward_layers(x=x)
         p_s, _ = time_steps(plain_step, 2, 1)
+        plain_bytes = B * 2 ** n * (8 + 16 * len(w.gates) + 8)
         extras["plain_updates_per_s"] = updates * 2 / p_s
         extras["plain_ms_per_step"] = p_s / 2 * 1e3
+        extras["plain_step_hbm_frac"] = plain_bytes / (p_s / 2) / 1e9 / HBM_PEAK_GBS
         extras["fused_over_plain"] = extras["plain_ms_per_step"] / ms_per_step
-        extras["fused_alg_bytes_per_step"] = bstep
-        extras["fused_step_hbm_frac"] = bstep / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS
-        extras["plain_step_hbm_frac"] = (B * 2 ** n * (8 + 16 * len(w.gates) + 8)) / (p_s / 2) / 1e9 / HBM_PEAK_GBS
-        # the bytes a gate-by-gate streaming path moves, over the fused step's time
-        extras["fused_per_gate_equivalent_hbm_frac"] = ((B * 2 ** n * (8 + 16 * len(w.gates) + 8)) / (ms_per_step / 1e3)
-                                                        / 1e9 / HBM_PEAK_GBS)
+        extras["fused_alg_bytes_per_step"] = sum(pbytes)
+        extras["fused_step_hbm_frac"] = sum(pbytes) / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS
 
-        # 24 qubits, batch 4 (same circuit family)
         log("24 qubits, batch 4")
-        w24 = workloads.rxryrz_ladder(n=24, B=4, depth=10, seed=1)
-        c24 = qandle.Circuit(w24.layers, num_qubits=24).to(dev)
-        x24 = w24.inputs["x"].to(dev)
-        with torch.no_grad():
-            settle(lambda: c24(x=x24))
-            t24, _ = time_steps(lambda: c24(x=x24), 5, 2)
-        p24 = fused.lower_circuit(c24.circuit)
-        np24 = len(executed_plan(p24, *fused.choose_shape(24)).passes)
-        _, b24 = fused_bytes_per_launch(4, 24, np24, stores_last=False)
-        extras["n24_updates_per_s"] = w24.amplitude_updates() * 5 / t24
-        extras["n24_ms_per_step"] = t24 / 5 * 1e3
-        extras["n24_hbm_frac"] = b24 / (t24 / 5) / 1e9 / HBM_PEAK_GBS
-        extras["n24_per_gate_equivalent_hbm_frac"] = (4 * 2 ** 24 * (8 + 16 * len(w24.gates) + 8)) / (t24 / 5) / 1e9 / HBM_PEAK_GBS
-        extras["n24_passes"] = np24
-
-        # adjoint forward + backward, 16 qubits, batch 256, 20 layers
-        log("adjoint forward + backward: 16 qubits, batch 256, 20 layers")
-        w3 = workloads.rxryrz_ladder(n=16, B=256, depth=20, seed=2, loss=True)
-        c3 = qandle.Circuit(w3.layers, num_qubits=16).to(dev)
-        x3 = w3.inputs["x"].to(dev).requires_grad_(True)
-        lw = w3.loss_w.to(dev)
-
-        def adj_step():
-            c3.zero_grad(set_to_none=True)
-            x3.grad = None
-            (c3(x=x3) * lw).sum().backward()
-
-        def plain_adj_step():
-            c3.zero_grad(set_to_none=True)
-            x3.grad = None
-            (c3.circuit.forward_layers(x=x3) * lw).sum().backward()
-
-        settle(adj_step)
-        ta, _ = time_steps(adj_step, 6, 2)
-        log(f"adjoint: {ta / 6 * 1e3:.2f} ms per forward + backward; plain per-gate autograd next")
-        tp, _ = time_steps(plain_adj_step, 1, 1)
-        extras["adjoint_fwd_bwd_ms"] = ta / 6 * 1e3
-        extras["plain_autograd_fwd_bwd_ms"] = tp * 1e3
-        extras["adjoint_speedup_vs_plain"] = tp / (ta / 6)
-        del c3, x3
+        extras.update(bench_n24(dev, time_steps, sync))
         torch.cuda.empty_cache()
-
-        # uneven batch: 2048 circuits, 4..18 qubits, depth 2..24, forward + adjoint in one graph
+        log("adjoint forward + backward: 16 qubits, batch 256, 20 layers")
+        extras.update(bench_adjoint(dev, time_steps, sync))
+        torch.cuda.empty_cache()
         log("uneven batch of 2048 circuits")
         extras.update(bench_uneven(dev, time_steps))
         torch.cuda.empty_cache()
-        # serving: Poisson arrivals at ~70% offered load of the measured fused throughput
         log("serving loop")
         extras.update(bench_serving(dev))
         torch.cuda.empty_cache()
+        if not args.no_cpu:
+            log("8-qubit configuration: dense reference path on the CPU beside the GPU forward")
+            extras.update(bench_c1(dev, time_steps, sync))
         result["extras"] = extras
-
         if not args.no_cpu:
             log("CPU baseline sample")
             result["cpu_baseline"] = cpu_baseline(w)
@@ -344,16 +430,76 @@ def main():
     print(json.dumps(result))
 
 
+def bench_n24(dev, time_steps, sync):
+    import torch
+
+    import qandle_amd as qandle
+    from qandle_amd import workloads
+    from qandle_amd._engine import fused
+
+    w = workloads.rxryrz_ladder(n=24, B=4, depth=10, seed=1)
+    c = qandle.Circuit(w.layers, num_qubits=24).to(dev)
+    x = w.inputs["x"].to(dev)
+
+    def fwd():
+        with torch.no_grad():
+            c(x=x)
+    settle(fwd, sync)
+    t, _ = time_steps(fwd, 5, 2)
+    prog = fused.lower_circuit(c.circuit)
+    npass = len(executed_plan(prog, *fused.choose_shape(24)).passes)
+    nbytes = sum(pass_bytes(4, 24, npass, stores_last=False))
+    return {"n24_updates_per_s": w.amplitude_updates() * 5 / t, "n24_ms_per_step": t / 5 * 1e3,
+            "n24_hbm_frac": nbytes / (t / 5) / 1e9 / HBM_PEAK_GBS, "n24_passes": npass}
+
+
+def bench_adjoint(dev, time_steps, sync):
+    import torch
+
+    import qandle_amd as qandle
+    from qandle_amd._engine import backward
+
+    w = adjoint_workload()
+    c3 = qandle.Circuit(w.layers, num_qubits=w.n).to(dev)
+    x3 = w.inputs["x"].to(dev).requires_grad_(True)
+    lw = w.loss_w.to(dev)
+
+    def adj_step():
+        c3.zero_grad(set_to_none=True)
+        x3.grad = None
+        (c3(x=x3) * lw).sum().backward()
+
+    def plain_adj_step():
+        c3.zero_grad(set_to_none=True)
+        x3.grad = None
+        (c3.circuit.forward_layers(x=x3) * lw).sum().backward()
+
+    settle(adj_step, sync)
+    ta, _ = time_steps(adj_step, 6, 2)
+    log(f"adjoint: {ta / 6 * 1e3:.2f} ms per forward + backward; adjoint kernel launches, then plain autograd")
+    events = []
+    backward.launch_events = events
+    try:
+        for _ in range(2):
+            adj_step()
+        sync()
+    finally:
+        backward.launch_events = None
+    roof = hbm_roofline(events, "qs_adj_<i> (adjoint sweep steps specialised per step program, hipRTC)")
+    tp, _ = time_steps(plain_adj_step, 1, 1)
+    out = {"adjoint_fwd_bwd_ms": ta / 6 * 1e3, "plain_autograd_fwd_bwd_ms": tp * 1e3,
+           "adjoint_speedup_vs_plain": tp / (ta / 6), "adjoint_roofline": roof}
+    del c3, x3
+    return out
+
+
 def bench_uneven(dev, time_steps):
     import torch
 
-    from qandle_amd import workloads
     from qandle_amd._engine import autotune
     from qandle_amd._engine.uneven import UnevenBatch
 
-    ws = workloads.random_uneven(2048, build_layers=False)
-    gen = torch.Generator().manual_seed(3)
-    lw = [torch.randn(w.B, w.n, generator=gen) for w in ws]
+    ws, lw = uneven_workloads()
 
     def make(R, TB, cost):
         ub = UnevenBatch.from_workloads(ws, dev, R=R, TB=TB, cost=cost)
@@ -365,6 +511,11 @@ def bench_uneven(dev, time_steps):
     tf, _ = time_steps(lambda: ub.replay(backward=False), 5, 1)
     tfb, _ = time_steps(lambda: ub.replay(backward=True), 5, 1)
     te, _ = time_steps(lambda: ub.run_forward(), 3, 1)
+    events = []
+    for _ in range(2):
+        ub.run_forward(events=events)
+    torch.cuda.synchronize(dev)
+    roof = hbm_roofline(events, f"qs_items_pass (level launches of (circuit pass, tile) work items; R={ub.R}, TB={ub.TB})")
     upd = ub.amplitude_updates
     return {
         "uneven_circuits": len(ws),
@@ -375,6 +526,7 @@ def bench_uneven(dev, time_steps):
         "uneven_fwd_eager_ms": te / 3 * 1e3,
         "uneven_levels": len(ub.fw_level_items),
         "uneven_autotune": table,
+        "uneven_roofline": roof,
     }
 
 
@@ -422,49 +574,72 @@ def bench_serving(dev, num=160, seed=4):
     }
 
 
+def bench_c1(dev, time_steps, sync):
+    """The 8-qubit configuration (batch 32, RX embedding, 4 x (RY per qubit + CNOT
+    ring), MeasureProbability): the reference's own cached-matrix path restated in
+    complex64 torch on the host (dense 2^n x 2^n gate matrices, the embedding's
+    einsum + matmul chain; oracle/qandle_c64.py, dense mode), median of 5 forwards
+    on the host's CPU share after the cache build, beside the fused GPU forward."""
+    import torch
+
+    import qandle_amd as qandle
+    from oracle import qandle_c64
+    from qandle_amd import workloads
+
+    w = workloads.embed_ry_ring(n=8, B=32, depth=4, seed=0)
+    threads = cpu_share()
+    torch.set_num_threads(threads)
+    ops = w.torch_ops()
+    t0 = time.perf_counter()
+    dense = qandle_c64.DenseCircuit(w.n, ops)
+    build_s = time.perf_counter() - t0
+    med, runs = median_time(lambda: dense(ops))
+    c = qandle.Circuit(w.layers, num_qubits=w.n).to(dev)
+    x = w.inputs["x"].to(dev)
+
+    def fwd():
+        with torch.no_grad():
+            c(x=x)
+    settle(fwd, sync)
+    tg, _ = time_steps(fwd, 50, 5)
+    upd = w.amplitude_updates()
+    return {"c1_workload": "8 qubits, batch 32, RX embedding + 4 x (RY per qubit + CNOT ring) + MeasureProbability",
+            "c1_cpu_dense_ms": med * 1e3, "c1_cpu_dense_runs_ms": [t * 1e3 for t in runs],
+            "c1_cpu_dense_build_s": build_s, "c1_cpu_threads": threads,
+            "c1_cpu_dense_updates_per_s": upd / med,
+            "c1_gpu_ms": tg / 50 * 1e3, "c1_gpu_updates_per_s": upd * 50 / tg,
+            "c1_gpu_over_cpu_dense": med / (tg / 50)}
+
+
 def cpu_baseline(w):
     """qandle's complex64 torch semantics restated (oracle/qandle_c64.py, structured
     mode: the dense reference needs 2 x 8 TiB of cached matrices per gate at 20
-    qubits) on a bounded sample: the full circuit on as many of the 64 samples as fit
-    in 10-30 s on this host (8 samples)."""
+    qubits) on a bounded sample of the headline workload: the full 810-gate circuit
+    on 4 of its 64 samples, median of 5 runs after a warm-up, on every CPU of this
+    process's share."""
     import torch
 
     from oracle import qandle_c64
 
-    threads = min(CPU_THREADS, os.cpu_count() or 1)
+    threads = cpu_share()
     torch.set_num_threads(threads)
-
-    def run(Bs):
-        ops = w.torch_ops()
-        ops[0] = (ops[0][0], ops[0][1], ops[0][2], ops[0][3][:Bs])
-        t0 = time.perf_counter()
-        qandle_c64.structured_run(w.n, ops, Bs)
-        return time.perf_counter() - t0
-
-    run(1)  # warm-up
-    Bs = 8  # 8 of 64 samples: 54 MB of states, well outside the caches; about 10-15 s here
-    dt = run(Bs)
+    Bs = 4
+    ops = w.torch_ops()
+    ops[0] = (ops[0][0], ops[0][1], ops[0][2], ops[0][3][:Bs])
+    med, runs = median_time(lambda: qandle_c64.structured_run(w.n, ops, Bs))
     updates = w.gate_ops * Bs * 2 ** w.n
     return {
-        "value": updates / dt,
+        "value": updates / med,
         "unit": "amplitude-gate updates/s",
         "cores": threads,
         "kind": "port",
         "sample": f"full {w.gate_ops}-gate circuit on {Bs} of the {w.B} samples, torch CPU complex64 "
-                  f"(structured restatement of qandle's gate semantics), {dt:.1f} s",
+                  f"(structured restatement of qandle's gate semantics), median of 5 runs "
+                  f"({', '.join(f'{t:.2f}' for t in runs)} s)",
         "cpu": _cpu_model(),
+        "host_cpus": os.cpu_count(),
+        "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
     }
-
-
-def _cpu_model():
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return "unknown"
 
 
 if __name__ == "__main__":

@@ -353,7 +353,7 @@ def adjoint_grads(n: int, ops, w: np.ndarray, psi0=None):
     Per gate g = exp(-i phi P / 2):  dL/dphi = 1/2 Im <lambda_g | P | psi_g>
     at the gate output; then psi, lambda <- U^dag psi, U^dag lambda.
     """
-    psi, _ = simulate(n, ops, psi0, record=True)
+    psi = simulate(n, ops, psi0)  # the sweep un-applies gates: no state is recorded
     B = psi.shape[0]
     w = np.broadcast_to(np.asarray(w, dtype=np.float64), (B, n))
     mask0 = np.zeros((n, 2 ** n))

@@ -29,6 +29,12 @@ from . import fused
 from . import lib as L
 
 
+# When set to a list, the backward sweep appends (start, end, algorithmic bytes)
+# for every adjoint kernel launch: HIP events recorded on the launch stream.
+# Measurement only (the benchmark's roofline of the adjoint kernels).
+launch_events = None
+
+
 def checkpoint_after(prog, pl, every_gates: int):
     """Pass indices after which the forward state is stored."""
     out = set()
@@ -119,8 +125,12 @@ class FusedCircuitFn(torch.autograd.Function):
                 ltab = torch.empty(idx.numel() * 16, dtype=torch.float32, device=psi.device)
                 L.check(lib.qs_lane_coeffs(L.ptr(ltab), L.ptr(utab), L.ptr(idx), idx.numel(), 1, st),
                         "qs_lane_coeffs")
+        events = launch_events
         for k, (w, nsub, lpos) in enumerate(ws.passes):
             seed_ptr = None if (k > 0 or seed is None) else L.ptr(seed)
+            if events is not None:
+                ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                ev[0].record(torch.cuda.current_stream(psi.device))
             if kern is not None:
                 kern.launch(k, L.ptr(psi), L.ptr(lam), L.ptr(utab), L.ptr(qtab), L.ptr(cs), L.ptr(gpart), seed_ptr,
                             B, NS, ntiles, st, None if ltab is None else L.ptr(ltab))
@@ -133,6 +143,10 @@ class FusedCircuitFn(torch.autograd.Function):
                     ),
                     "qs_adjoint_pass",
                 )
+            if events is not None:
+                ev[1].record(torch.cuda.current_stream(psi.device))
+                # psi and lambda are read and written once per step; a seeded step reads psi only
+                events.append((ev[0], ev[1], B * (2 ** n) * (24 if seed_ptr is not None else 32)))
             i = npass - 1 - k  # forward pass this step undid
             if k < nmain and (i - 1) in ckpts:
                 psi.copy_(ckpts[i - 1])

@@ -334,15 +334,37 @@ class UnevenBatch:
                                            L.ptr(ws.recipes), L.ptr(ws.offsets),
                                            ws.nrec, L.ptr(self.cs), st), "qs_compose_tables")
 
-    def run_forward(self):
+    def level_bytes(self):
+        """Algorithmic HBM bytes of each forward level launch: a circuit's first pass
+        writes its generated state (8 B per amplitude), every later pass reads and
+        writes it (16 B); the measurement partials are negligible."""
+        out = []
+        for lvl in range(len(self.fw_level_items)):
+            tot = 0
+            for c in self.circs:
+                if lvl < len(c.plan.passes):
+                    tot += c.B * (2 ** c.n) * (8 if lvl == 0 else 16)
+            out.append(tot)
+        return out
+
+    def run_forward(self, events=None):
+        """Forward of every circuit.  ``events``: optional list; (start, end, bytes)
+        HIP events on the launch stream are appended around every level launch."""
         lib, st = self.lib, self._stream()
         L.check(lib.qs_half_angle_coeffs(L.ptr(self.cs), L.ptr(self.phi), self.phi.numel(), st), "coeffs")
         L.check(lib.qs_factor_entries(L.ptr(self.efac), L.ptr(self.x), L.ptr(self.entries), self.nentries, st),
                 "qs_factor_entries")
         self._compose(self.fw_tables, self.fw_utab, None, st)
-        for items, cnt in zip(self.fw_levels, self.fw_level_items):
+        nbytes = self.level_bytes() if events is not None else None
+        for lvl, (items, cnt) in enumerate(zip(self.fw_levels, self.fw_level_items)):
+            if events is not None:
+                ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                ev[0].record(torch.cuda.current_stream(self.device))
             L.check(lib.qs_items_pass(L.ptr(self.fw_descs), L.ptr(items), cnt, self.R, self.TB, self.grid, st),
                     "qs_items_pass")
+            if events is not None:
+                ev[1].record(torch.cuda.current_stream(self.device))
+                events.append((ev[0], ev[1], nbytes[lvl]))
         L.check(lib.qs_items_measure_finish(L.ptr(self.m_descs), len(self.circs), self.max_n, st),
                 "qs_items_measure_finish")
 

@@ -1,0 +1,160 @@
+"""Writes FP64 reference data for the benchmark workloads at their full sizes.
+
+    python tests/golden/make_config_fixtures.py [c2 c3 c4 c5]
+
+The GPU box only compares against these files: every expected value here comes
+from the FP64 oracle (oracle/fp64.py, itself pinned by tests/test_oracle.py),
+computed in the build container.  Each file also stores the inputs and angles
+it was computed from, so a GPU test can check that the seeded workload
+generators still produce the same circuit before it compares results.
+
+* c2_rows.npz  -- 20 qubits, batch 64, 10 x (RX, RY, RZ per qubit + CNOT
+  ladder), RX embedding: MeasureProbability of four sampled rows.
+* c3_grads.npz -- 16 qubits, batch 256, 20 layers of the same family, loss
+  sum(w * P): dL/dtheta summed over all 256 samples (960 angles), dL/dx
+  (256 x 16) and P (256 x 16).
+* c4_subset.npz -- the 2048-circuit uneven mix: probabilities and gradients of
+  every circuit with 17-18 qubits and depth >= 20, plus a spread of others.
+* c5_rows.npz  -- serving requests of 16-24 qubits: probabilities of the first
+  and last row of every request of at most 20 qubits.
+"""
+import multiprocessing as mp
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import fp64  # noqa: E402
+
+# serving requests checked at their full sizes: (qubits, batch, depth, generator seed)
+C5_REQUESTS = [(16, 128, 6, 501), (18, 64, 10, 502), (20, 37, 4, 503), (22, 100, 5, 504), (24, 16, 8, 505),
+               (17, 90, 16, 506)]
+C4_SPREAD = [0, 1, 2, 100, 511, 777, 1024, 1500, 2000, 2047]
+
+
+def _ops(x, gates, n):
+    ops = [("embed", "x", list(range(n)), np.asarray(x, dtype=np.float64))]
+    for g in gates:
+        ops.append(("cnot", g[1][0], g[1][1]) if g[0] == "cnot" else (g[0], g[1], float(g[2])))
+    return ops
+
+
+def _thetas(gates):
+    return np.array([g[2] for g in gates if g[0] != "cnot"], dtype=np.float32)
+
+
+def c2():
+    from qandle_amd import workloads
+
+    w = workloads.rxryrz_ladder(n=20, B=64, depth=10, seed=1)
+    rows = [0, 17, 42, 63]
+    X = w.inputs["x"].numpy()
+    P = np.stack([fp64.measure_prob(fp64.simulate(20, _ops(X[r:r + 1], w.gates, 20)), 20)[0] for r in rows])
+    return {"x": X, "theta": _thetas(w.gates), "rows": np.array(rows), "P": P}
+
+
+def _c3_chunk(args):
+    lo, hi = args
+    from qandle_amd import workloads
+
+    w = workloads.rxryrz_ladder(n=16, B=256, depth=20, seed=2, loss=True)
+    X = w.inputs["x"].numpy()[lo:hi]
+    ops = _ops(X, w.gates, 16)
+    wt = w.loss_w.double().numpy()[lo:hi]
+    P = fp64.measure_prob(fp64.simulate(16, ops), 16)
+    g = fp64.adjoint_grads(16, ops, wt)
+    gth = np.array([a for a, op in zip(g[1:], ops[1:]) if op[0] != "cnot"])
+    return lo, P, g[0], gth
+
+
+def c3(workers=8):
+    from qandle_amd import workloads
+
+    w = workloads.rxryrz_ladder(n=16, B=256, depth=20, seed=2, loss=True)
+    step = 256 // workers
+    with mp.get_context("spawn").Pool(workers) as pool:
+        parts = pool.map(_c3_chunk, [(lo, lo + step) for lo in range(0, 256, step)])
+    parts.sort(key=lambda t: t[0])
+    return {"x": w.inputs["x"].numpy(), "theta": _thetas(w.gates), "w": w.loss_w.numpy(),
+            "P": np.concatenate([p[1] for p in parts]), "grad_x": np.concatenate([p[2] for p in parts]),
+            "grad_theta": np.sum([p[3] for p in parts], axis=0)}
+
+
+def c4_selection(ws):
+    big = [i for i, w in enumerate(ws) if w.n >= 17 and sum(g[0] != "cnot" for g in w.gates) // w.n >= 20]
+    return sorted(set(big) | set(C4_SPREAD))
+
+
+def _c4_one(args):
+    i, wt = args
+    from qandle_amd import workloads
+
+    w = workloads.random_uneven(2048, build_layers=False)[i]
+    ops = _ops(w.inputs["x"].numpy(), w.gates, w.n)
+    P = fp64.measure_prob(fp64.simulate(w.n, ops), w.n)[0]
+    g = fp64.adjoint_grads(w.n, ops, wt)
+    gth = np.array([a for a, op in zip(g[1:], ops[1:]) if op[0] != "cnot"])
+    return i, P, g[0][0], gth
+
+
+def c4(workers=8):
+    import torch
+
+    from qandle_amd import workloads
+
+    ws = workloads.random_uneven(2048, build_layers=False)
+    gen = torch.Generator().manual_seed(3)  # the benchmark's loss weights, one (1, n) draw per circuit
+    lw = [torch.randn(w.B, w.n, generator=gen).double().numpy() for w in ws]
+    sel = c4_selection(ws)
+    with mp.get_context("spawn").Pool(workers) as pool:
+        res = pool.map(_c4_one, [(i, lw[i]) for i in sel])
+    out = {"index": np.array(sel)}
+    for i, P, gx, gth in res:
+        w = ws[i]
+        out[f"n{i}"] = np.array(w.n)
+        out[f"x{i}"] = w.inputs["x"].numpy()[0]
+        out[f"theta{i}"] = _thetas(w.gates)
+        out[f"w{i}"] = lw[i][0]
+        out[f"P{i}"] = P
+        out[f"gx{i}"] = gx
+        out[f"gth{i}"] = gth
+    return out
+
+
+def c5_request(n, B, depth, seed):
+    """The serving request workload (the hardware-efficient ansatz) of one entry of C5_REQUESTS."""
+    import torch
+
+    from qandle_amd import workloads
+
+    return workloads.hardware_efficient(n, B, depth, torch.Generator().manual_seed(seed))
+
+
+def c5():
+    out = {}
+    for k, (n, B, depth, seed) in enumerate(C5_REQUESTS):
+        w = c5_request(n, B, depth, seed)
+        X = w.inputs["x"].numpy()
+        out[f"x{k}"] = X
+        out[f"theta{k}"] = _thetas(w.gates)
+        if n <= 20:
+            rows = [0, B - 1]
+            out[f"rows{k}"] = np.array(rows)
+            out[f"P{k}"] = np.stack([fp64.measure_prob(fp64.simulate(n, _ops(X[r:r + 1], w.gates, n)), n)[0]
+                                     for r in rows])
+    return out
+
+
+if __name__ == "__main__":
+    which = sys.argv[1:] or ["c2", "c3", "c4", "c5"]
+    makers = {"c2": (c2, "c2_rows.npz"), "c3": (c3, "c3_grads.npz"), "c4": (c4, "c4_subset.npz"),
+              "c5": (c5, "c5_rows.npz")}
+    for name in which:
+        fn, fname = makers[name]
+        data = fn()
+        np.savez_compressed(os.path.join(HERE, fname), **data)
+        print(f"{fname}: {sum(np.asarray(v).nbytes for v in data.values()) / 1e6:.2f} MB raw", flush=True)
