@@ -714,9 +714,11 @@ def use_jit(n: int, R: int, TB: int) -> bool:
     return bool(getattr(config, "SPECIALISE", False))
 
 
-def specialised_kernels(prog: Program, pl: PL.Plan, ws: WordSet, B: int, gen_first: bool, fold: bool = False):
+def specialised_kernels(prog: Program, pl: PL.Plan, ws: WordSet, B: int, gen_first: bool, fold: bool = False,
+                        load: bool = True):
     """Run-time compiled kernels of this plan's passes (cached on the program);
-    with ``fold`` the last pass writes readout partials (readout_fold)."""
+    with ``fold`` the last pass writes readout partials (readout_fold).
+    ``load`` False only compiles them into the code-object cache."""
     from .. import config
     from . import jit
 
@@ -725,9 +727,33 @@ def specialised_kernels(prog: Program, pl: PL.Plan, ws: WordSet, B: int, gen_fir
     hit = prog.words_cache.get(("jit",) + key)
     if hit is None:
         masks = readout_fold(prog)[1] if fold else None
-        hit = jit.ProgramKernels(ws.passes, prog.n, pl.R, pl.TB, gen_first, prog.measure, readout=masks, **opts)
-        prog.words_cache[("jit",) + key] = hit
+        hit = jit.ProgramKernels(ws.passes, prog.n, pl.R, pl.TB, gen_first, prog.measure, readout=masks,
+                                 load=load, **opts)
+        if load:
+            prog.words_cache[("jit",) + key] = hit
     return hit
+
+
+def forward_kernels(prog: Program, B: int, R: int, TB: int, cm: PL.CostModel = None, device="cpu",
+                    load: bool = True):
+    """The compiled kernels a measured forward without gradients from a product
+    state runs (the program ``execute`` ends up with: leading rotations absorbed,
+    trailing CNOTs folded into the readout), compiled now if the code-object cache
+    does not hold them.  ``device`` only places the encoded words; the kernel
+    sources depend on the program's structure, not on its angles, batch or device,
+    so a worker process can build them ahead of a server that runs them later
+    (``load`` False: compile into the code-object cache without loading them
+    onto a device)."""
+    from .. import config
+
+    if getattr(config, "ABSORB_LEADING_ROTATIONS", True) and prog.init[0] in ("embed", "input"):
+        lead = leading_rotations(prog)
+        if lead is not None:
+            prog = lead[0]
+    fold = folds(prog, True)
+    pl = get_plan(prog, R, TB, cm, fold)
+    ws = pass_words(prog, pl, B, device)
+    return specialised_kernels(prog, pl, ws, B, True, fold, load=load)
 
 
 def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,

@@ -46,6 +46,8 @@ struct JArgs {
   i64 B; i64 ntiles; int store; int pad;
 };
 __device__ __forceinline__ int swz(int l) { int f = (l >> 5) ^ (l >> 10) ^ (l >> 15); return l ^ (f & 31); }
+// the swizzle with bit 0 left alone: an amplitude pair keeps one 16-byte slot in order
+__device__ __forceinline__ int swz_e(int l) { int f = (l >> 5) ^ (l >> 10) ^ (l >> 15); return l ^ (f & 30); }
 // a tile element by byte offset: with the offset kept as (index << 3) the swizzled address of
 // register j is one XOR with a constant, (lb << 3) ^ (stride_j << 3)
 __device__ __forceinline__ float2 &lds_at(float2 *base, int byte_off) {
@@ -226,8 +228,8 @@ OPTIONS = ["--offload-arch=gfx950", "-O3", "-std=c++17",
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-def _swz(l: int) -> int:
-    return PL.swz(l)
+def _swz(l: int, mask: int = 31) -> int:
+    return PL.swz(l, mask)
 
 
 def lane_entries(words, R: int, TB: int, multi: bool):
@@ -380,6 +382,135 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
     return "\n".join(o)
 
 
+def async_load_offsets(lpos, R: int, TB: int):
+    """Amplitude offsets of the direct tile loads of pass_source_async.
+
+    16-byte LDS slot s of a tile buffer holds tile elements swz_e(2s) and
+    swz_e(2s) + 1 (the pair swz_e keeps in order); load ii of thread tid fills
+    slot s = ii * 2^TB + tid (lane j of a wave's 64-lane load lands in slot j of
+    its 1 KiB block).  Its global amplitude offset is scatter(swz_e(2s)), where
+    scatter puts tile element bit k at amplitude bit lpos[k].  Both maps are
+    linear over XOR, so the offset is the XOR of per-thread-bit constants
+    ``d_bit[b]`` (bit b of tid set) and a per-load constant ``d_inst[ii]``."""
+    K = R + TB
+    T = 1 << TB
+
+    def scatter(l):
+        return sum(1 << lpos[k] for k in range(K) if (l >> k) & 1)
+
+    d_bit = [scatter(_swz(2 << b, 30)) for b in range(TB)]
+    d_inst = [scatter(_swz(2 * ii * T, 30)) for ii in range(1 << (R - 1))]
+    return d_bit, d_inst
+
+
+def pass_source_async(name: str, words, n: int, R: int, TB: int, lpos, measure: bool, waves: int = 2,
+                      readout=None, lanes="lds", lane_off: int = 0, interleave: int = 4,
+                      wave_sync: bool = True) -> str:
+    """HIP source of one specialised forward pass kernel whose HBM reads overlap
+    its arithmetic: one persistent workgroup per compute unit holds two tile
+    buffers in LDS (2 x 2^(R+TB) amplitudes).  While the sub-passes work on the
+    tile in one buffer, the next tile streams from HBM straight into the other
+    with gfx950's direct global->LDS loads (``global_load_lds_dwordx4``: no
+    registers, no LDS write instructions); the finished tile is stored from LDS
+    through registers as in pass_source.  The two buffers are separate LDS
+    arrays and the tile loop is unrolled by two, so that the compiler can tell
+    the buffer being loaded from the one being read and does not wait for the
+    loads before every LDS read.  The tile layout keeps amplitude pairs in order
+    in one 16-byte slot (swizzle ``swz_e``), so that lane j of a 64-lane load
+    lands in slot j of its 1 KiB block.  Tiles within one sample only
+    (n >= R + TB), and not the first pass of a program (it generates its tile
+    rather than loading it).  Same sub-pass code, tables and partial layouts as
+    pass_source: results are the same bit for bit."""
+    K = R + TB
+    T, NR = 1 << TB, 1 << R
+    if n < K:
+        raise ValueError("asynchronous tile loads need tiles within one sample")
+    subs = PL.decode_pass(words, R, TB)
+    lpos = [int(x) for x in lpos]
+    npos = [p for p in range(n) if p not in lpos]
+    nn = len(npos)
+
+    o = []
+    e = o.append
+    wattr = f" __attribute__((amdgpu_waves_per_eu({waves})))" if waves else ""
+    e(f'extern "C" __global__ void __launch_bounds__({T}){wattr}')
+    e(f"{name}(JArgs A) {{")
+    e(f"  __shared__ __align__(16) float2 buf0[{1 << K}];")
+    e(f"  __shared__ __align__(16) float2 buf1[{1 << K}];")
+    if measure and TB > 6:
+        e(f"  __shared__ float red[{(T // 64) * ((K + 1) if readout is None else (n + 1))}];")
+    lane_k = None
+    if lanes == "lds":
+        ents = lane_entries(words, R, TB, False)
+        lane_k = {ui: k for k, ui in enumerate(ents)}
+        if ents:
+            e(f"  __shared__ __align__(16) float4 ctab[{4 * len(ents)}];")
+    e("  const int tid = threadIdx.x;")
+    e("  i64 g_tid = 0;")
+    for i in range(TB):
+        e(f"  g_tid |= (i64)((tid >> {i}) & 1) << {lpos[i + 1]};")
+    goff = [sum(1 << lpos[TB + 1 + i] for i in range(R - 1) if (it >> i) & 1) for it in range(NR // 2)]
+    d_bit, d_inst = async_load_offsets(lpos, R, TB)
+    e("  i64 d_tid = 0;")
+    for b in range(TB):
+        e(f"  d_tid ^= ((tid >> {b}) & 1) ? (i64){d_bit[b]}LL : 0LL;")
+    e("  auto tile_base = [&](i64 t) -> i64 {")
+    e(f"    i64 b = (t >> {nn}) << {n};")
+    for i, p in enumerate(npos):
+        e(f"    b |= ((t >> {i}) & 1) << {p};")
+    e("    return b;")
+    e("  };")
+    e("  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);")
+    for bsel in (0, 1):
+        e(f"  auto issue{bsel} = [&](i64 tt) {{")
+        e("    const i64 b = tile_base(tt);")
+        e("    const float4 *src = reinterpret_cast<const float4 *>(A.src);")
+        for ii in range(NR // 2):
+            e(f"    __builtin_amdgcn_global_load_lds(src + ((b | (d_tid ^ {d_inst[ii]}LL)) >> 1),")
+            e(f"        (__attribute__((address_space(3))) void *)(buf{bsel} + 2 * ({ii * T} + 64 * wave)), 16, 0, 0);")
+        e("  };")
+    if lane_k:
+        nl = 4 * len(lane_k)
+        e(f"  for (int c = tid; c < {nl}; c += {T}) ctab[c] = reinterpret_cast<const float4 *>(A.ltab)[{4 * lane_off} + c];")
+    # one tile: its sub-passes on `tile` and its store
+    e("  auto work = [&](float2 *tile, const i64 t) __attribute__((always_inline)) {")
+    e("    const i64 base = tile_base(t);")
+    e(f"    const i64 block_sample = base >> {n};")
+    e(f"    float2 v[{NR}];")
+    for si, sub in enumerate(subs):
+        _sub_pass(e, sub, si, len(subs), n, R, TB, False, False, measure and si == len(subs) - 1, readout, 0,
+                  lanes=lane_k if lanes == "lds" else lanes, interleave=interleave,
+                  wave_sync=wave_sync and si + 1 < len(subs) and wave_local(sub, subs[si + 1], TB), swz_mask=30)
+    e("    if (A.store) {")
+    e("      const int tz = pinv(tid);")
+    for it in range(NR // 2):
+        e(f"      {{ const i64 gi = base | g_tid | {goff[it]}LL; const int sl = swz_e(2 * ({it * T} + tz));")
+        e("        reinterpret_cast<float4 *>(A.state)[gi >> 1] = *reinterpret_cast<const float4 *>(&tile[sl]); }")
+    e("    }")
+    e("  };")
+    # the loads of a tile (and this wave's earlier stores) complete, then every wave's; the buffer
+    # refilled next was last read by the store phase before that barrier
+    # s_waitcnt vmcnt(0) (expcnt and lgkmcnt unconstrained) through the builtin, so that the
+    # compiler's own wait tracking knows the buffer's loads have landed
+    wait = "    __builtin_amdgcn_s_waitcnt(0x0F70); __syncthreads();"
+    e("  const i64 G = gridDim.x;")
+    e("  i64 t = blockIdx.x;")
+    e("  if (t < A.ntiles) issue0(t);")
+    e("  while (t < A.ntiles) {")
+    e(wait)
+    e("    if (t + G < A.ntiles) issue1(t + G);")
+    e("    work(buf0, t);")
+    e("    t += G;")
+    e("    if (t >= A.ntiles) break;")
+    e(wait)
+    e("    if (t + G < A.ntiles) issue0(t + G);")
+    e("    work(buf1, t);")
+    e("    t += G;")
+    e("  }")
+    e("}")
+    return "\n".join(o)
+
+
 def pass_source_pipelined(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, measure: bool,
                           waves: int = None, readout=None, loaders: int = 8) -> str:
     """HIP source of one specialised forward pass kernel with a load/compute
@@ -487,7 +618,7 @@ def wave_local(a, b, TB: int) -> bool:
 
 
 def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu_pairs=0, pipe=None, lanes=False,
-              interleave=False, wave_sync=False):
+              interleave=False, wave_sync=False, swz_mask=31):
     """One sub-pass.  ``pipe`` (pipelined kernels): a dict; the tile is
     ``buf[boff + i]``, no closing barrier is emitted, and a measurement's
     cross-wave finish is left in ``pipe["finish"]`` for after the caller's
@@ -510,10 +641,10 @@ def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu
     e(f"      const i64 sample = {'gbase >> ' + str(n) if multi else 'block_sample'};")
     e("      const bool live = sample < A.B;")
     e("      const i64 csample = live ? sample : 0;")
-    e("      const int lb = swz(gl);")
+    e(f"      const int lb = {'swz' if swz_mask == 31 else 'swz_e'}(gl);")
     if pipe is None:
         e("      const int lb8 = lb << 3;")
-    sw = [_swz(1 << sub.P[r]) for r in range(R)]
+    sw = [_swz(1 << sub.P[r], swz_mask) for r in range(R)]
     swj = []
     for j in range(NR):
         a = 0
@@ -1173,7 +1304,8 @@ class ProgramKernels:
     def __init__(self, passes, n: int, R: int, TB: int, gen_first: bool, measure: bool,
                  prefetch: bool = False, waves: int = 2, readout=None, valu_pairs: int = 0,
                  grid: str = "full", pipeline: int = 0, lanes="lds", split_pairs: bool = True,
-                 interleave: int = 4, wave_sync: bool = True, stagger: int = 0):
+                 interleave: int = 4, wave_sync: bool = True, stagger: int = 0, load: bool = True,
+                 async_loads: bool = False):
         self.n, self.R, self.TB = n, R, TB
         self.pipelined = bool(pipeline) and n >= R + TB
         self.block = (1 << TB) + ((1 << pipeline) if self.pipelined else 0)
@@ -1181,13 +1313,19 @@ class ProgramKernels:
         npass = len(passes)
         self.sources, self.names = [], []
         self.lane_index = []  # utab index of every lane-table entry (lanes == "lds"), pass by pass
+        self.persistent = []  # per pass: True when its kernel walks tiles with a resident grid
         for i, (w, _nsub, lpos) in enumerate(passes):
             words = w.cpu().numpy() if hasattr(w, "cpu") else w
             nm = f"{KERNEL}_{i}"
             lane_off = len(self.lane_index)
             if lanes == "lds" and not self.pipelined:
                 self.lane_index += lane_entries(words, R, TB, n < R + TB)
-            if self.pipelined:
+            use_async = async_loads and not self.pipelined and n >= R + TB and not (gen_first and i == 0)
+            self.persistent.append(use_async)
+            if use_async:
+                self.sources.append(pass_source_async(nm, words, n, R, TB, lpos, measure and i == npass - 1, waves,
+                                                      readout, lanes, lane_off, interleave, wave_sync))
+            elif self.pipelined:
                 self.sources.append(pass_source_pipelined(nm, words, n, R, TB, lpos, gen_first and i == 0,
                                                           measure and i == npass - 1, waves, readout, pipeline))
             else:
@@ -1195,7 +1333,11 @@ class ProgramKernels:
                                                 measure and i == npass - 1, prefetch, waves, readout, valu_pairs,
                                                 lanes, split_pairs, lane_off, interleave, wave_sync, stagger))
             self.names.append(nm)
-        self.funcs = kernels_for(self.sources, self.names)
+        if load:
+            self.funcs = kernels_for(self.sources, self.names)
+        else:  # compile into the code-object cache only (a process that launches nothing)
+            _code_for("\n\n".join(self.sources))
+            self.funcs = []
         self.grid_cap = [None] * len(self.funcs)
         self.lanes = lane_mode(lanes, self.pipelined)
         self._lane_idx = {}
@@ -1216,7 +1358,7 @@ class ProgramKernels:
     def launch(self, i: int, state, src, utab, cs, efac, mpart, B: int, ntiles: int, store: int, stream,
                ltab=None):
         lib = L.load()
-        if self.full_grid:
+        if self.full_grid and not self.persistent[i]:
             grid = max(1, min(ntiles, 0x7fffffff))
         else:
             if self.grid_cap[i] is None:

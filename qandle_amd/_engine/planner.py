@@ -299,10 +299,11 @@ def _plan_passes(n: int, gates, R: int, TB: int, cm: CostModel, modes) -> Plan:
     return Plan(n, R, TB, passes)
 
 
-def swz(l: int) -> int:
-    """LDS slot of tile element l (same XOR-linear map as the kernels' swz)."""
+def swz(l: int, mask: int = 31) -> int:
+    """LDS slot of tile element l (same XOR-linear map as the kernels' swz;
+    ``mask`` 30 keeps bit 0, so an amplitude pair stays in order in one 16-byte slot)."""
     f = (l >> 5) ^ (l >> 10) ^ (l >> 15)
-    return l ^ (f & 31)
+    return l ^ (f & mask)
 
 
 def plan_stats(pl: Plan, gates, cm: CostModel = None):

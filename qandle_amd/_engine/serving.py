@@ -3,29 +3,41 @@
 The reference has no serving component; this is the request path a model
 server puts in front of ``Circuit.forward``:
 
-* Requests (a lowered circuit, its inputs, an arrival time) wait in a FIFO.
-* Admission is by HBM footprint: a request is admitted when the resident
-  bytes of everything in flight plus its own state (B x 2^n x 8 B, plus its
-  measurement partials) stay under the cap and a stream slot is free.  A
-  request whose batch alone exceeds the cap is split into batch chunks that
-  each fit; a single sample that does not fit is rejected with an error.
+* Requests (a lowered circuit, its inputs, an arrival time) wait in a queue.
+* Admission is by HBM footprint: a chunk of a request is admitted when the
+  resident bytes of everything in flight plus its own state (B x 2^n x 8 B,
+  plus its measurement partials and factors) stay under the cap and an
+  in-flight slot is free.  A request whose batch alone exceeds the cap is
+  split into batch chunks that each fit; a single sample that does not fit is
+  rejected with an error.
+* Order: the smallest waiting chunk first (shortest-footprint first keeps small
+  requests from queueing behind large ones), except that chunks which have
+  waited longer than ``max_wait_s`` go first, in arrival order.  When such an
+  aged chunk does not fit yet, memory is reserved for it: other chunks are only
+  admitted while they leave room for it (backfilling), so it is admitted as
+  soon as enough in-flight work retires and nothing starves.
 * Admitted work is enqueued at once on one of several HIP streams (default
   priority) and overlaps on the device with everything else in flight; the
   host never blocks on a request: completion is detected by polling a HIP
   event, after which the request's memory is released and more requests are
   admitted (continuous batching, not a queue drained one batch at a time).
-* Results are a pure function of (circuit, inputs): the kernel shape and
-  fusion plan do not depend on load, and every reduction has a fixed order, so
-  a request computes the same bits alone and in a mix.
-* Requests run on the library's interpreting pass kernels: a new circuit is
-  served at once instead of waiting for per-program kernels to compile.
-* Latency = completion time - arrival time (host clock); p50/p99 reported with
-  throughput.
+* Kernels: requests of at least ``compiled_from_qubits`` qubits run the pass
+  kernels compiled for their program structure (``precompile`` builds them
+  ahead of serving, in worker processes, into the code-object cache); smaller
+  requests, which are bound by launch overhead rather than by the kernels,
+  run the library's interpreting kernels and never wait for a compiler.  The
+  choice is a function of the request alone.
+* Results are a pure function of (circuit, inputs): the kernel choice, shape
+  and fusion plan do not depend on load or chunking, and each sample's tiles
+  are reduced in a fixed order, so a request computes the same bits alone, in
+  a mix, and split into chunks.
+* Latency = completion time - arrival time (host clock); service time =
+  completion - first admission.  p50/p99 of both are reported with throughput.
 """
 from __future__ import annotations
 
-import collections
 import dataclasses
+import multiprocessing as mp
 import time
 import typing
 
@@ -46,6 +58,7 @@ class Request:
     n: int = 0
     B: int = 0
     footprint: int = 0
+    compiled: bool = False
     chunks: list = dataclasses.field(default_factory=list)
     parts: dict = dataclasses.field(default_factory=dict)
     result: typing.Optional[torch.Tensor] = None
@@ -57,6 +70,10 @@ class Request:
     @property
     def latency(self) -> float:
         return self.t_done - self.arrival
+
+    @property
+    def service(self) -> float:
+        return self.t_done - self.t_admit
 
 
 @dataclasses.dataclass
@@ -71,20 +88,87 @@ class _Chunk:
 
 
 def state_footprint(n: int, B: int, R: int, TB: int) -> int:
-    """HBM bytes a running evaluation of (n, B) holds: the state plus partials."""
+    """HBM bytes a running evaluation of (n, B) holds: the state plus partials
+    (either partial layout) and factor tables."""
     lib = L.load()
-    return B * (2 ** n) * 8 + max(lib.qs_fused_measure_work_bytes(B, n, R, TB), 0) + B * n * 32 + 4096
+    part = max(lib.qs_fused_measure_work_bytes(B, n, R, TB), lib.qs_readout_work_bytes(B, n, R, TB), 0)
+    return B * (2 ** n) * 8 + part + B * n * 32 + 4096
+
+
+def kernel_shape(n: int, compiled: bool):
+    """(R, TB) of a request's passes: the compiled forward shape, or the
+    interpreting kernels' one-wave tiles."""
+    return fused.choose_shape(n) if compiled else fused.interpreted_shape(n)
+
+
+def _structure(prog):
+    """Hashable structure of a lowered program: what its compiled kernels depend on."""
+    gates = tuple((g.kind, g.pos, g.ctrl) for g in prog.gates)
+    return prog.n, gates, tuple(map(str, prog.init[:3])), prog.measure
+
+
+def _spec(prog):
+    """(n, gate spec, embed spec) of a lowered program, for fused.program_from_spec
+    (angles are left out: the kernels do not depend on them)."""
+    names = {fused.PL.RX: "rx", fused.PL.RY: "ry", fused.PL.RZ: "rz"}
+    n = prog.n
+    gates = [("cnot", (n - 1 - g.ctrl, n - 1 - g.pos)) if g.kind == fused.PL.CNOT
+             else (names[g.kind], n - 1 - g.pos, 0.0) for g in prog.gates]
+    embed = None
+    if prog.init[0] == "embed":
+        embed = ("xyz"[prog.init[1]], [n - 1 - b for b in prog.init[2]], prog.init[3])
+    return n, gates, embed
+
+
+def _compile_worker(specs):
+    """Compile (into the on-disk code-object cache; nothing is loaded onto a
+    device) the forward kernels of each (n, gate spec, embed spec) program."""
+    for n, gates, embed in specs:
+        prog = fused.program_from_spec(n, gates, embed)
+        R, TB = kernel_shape(n, True)
+        fused.forward_kernels(prog, 1, R, TB, fused.DEFAULT_COST, "cpu", load=False)
+    return len(specs)
+
+
+def precompile(requests, compiled_from_qubits: int, workers: int = 8) -> int:
+    """Compile the pass kernels of every distinct program structure among the
+    requests that will run compiled kernels, in ``workers`` processes (hipRTC
+    compiles one program at a time per process).  Angles are data, so one
+    structure's kernels serve every request of that structure.  Returns the
+    number of structures."""
+    seen, specs = set(), []
+    for r in requests:
+        prog = fused.lower_circuit(r.circuit.circuit)
+        if prog is None or prog.n < compiled_from_qubits:
+            continue
+        key = _structure(prog)
+        if key not in seen:
+            seen.add(key)
+            specs.append(_spec(prog))
+    if not specs:
+        return 0
+    workers = max(1, min(workers, len(specs)))
+    # largest programs first, dealt round-robin so the workers finish together
+    specs.sort(key=lambda s: -(len(s[1]) << s[0]))
+    groups = [specs[i::workers] for i in range(workers)]
+    if workers == 1:
+        _compile_worker(groups[0])
+    else:
+        with mp.get_context("spawn").Pool(workers) as pool:
+            pool.map(_compile_worker, groups)
+    return len(specs)
 
 
 class ServingEngine:
     def __init__(self, device, cap_bytes: float = 200e9, streams: int = 4, max_inflight: int = 64,
-                 max_bypass_s: float = 2.0):
+                 max_wait_s: float = 2.0, compiled_from_qubits: typing.Optional[int] = None):
         self.device = torch.device(device)
-        self.max_bypass_s = max_bypass_s
+        self.max_wait_s = max_wait_s
         self.cap = int(cap_bytes)
+        self.compiled_from = compiled_from_qubits
         self.streams = [torch.cuda.Stream(self.device) for _ in range(streams)]
         self.max_inflight = max_inflight
-        self.queue: collections.deque = collections.deque()
+        self.queue: typing.List[_Chunk] = []
         self.inflight: typing.List[_Chunk] = []
         self.resident = 0
         self.peak_resident = 0
@@ -95,21 +179,28 @@ class ServingEngine:
 
     # ----------------------------------------------------------- admission
     def prepare(self, req: Request):
-        """Lower + plan (once, before serving) and split over-cap batches."""
+        """Lower + plan (and, for compiled requests, load their kernels) before
+        serving; split over-cap batches into chunks."""
         prog = fused.lower_circuit(req.circuit.circuit)
         if prog is None or not prog.measure:
             raise ValueError("serving takes circuits that end in MeasureProbability")
         x = req.inputs[prog.init[3]] if prog.init[0] == "embed" else None
         req.n = prog.n
         req.B = 1 if x is None or x.dim() == 1 else x.shape[0]
-        R, TB = fused.interpreted_shape(prog.n)
-        fused.get_plan(prog, R, TB, fused.INTERPRETED_COST)
+        req.compiled = self.compiled_from is not None and prog.n >= self.compiled_from
+        R, TB = kernel_shape(prog.n, req.compiled)
+        if req.compiled:
+            fused.forward_kernels(prog, 1, R, TB, fused.DEFAULT_COST, self.device)
+        else:
+            fused.get_plan(prog, R, TB, fused.INTERPRETED_COST)
         per = state_footprint(req.n, 1, R, TB)
         req.footprint = state_footprint(req.n, req.B, R, TB)
         if per > self.cap:
             req.error = f"one sample needs {per} bytes, over the {self.cap}-byte admission cap"
             return req
         step = req.B if req.footprint <= self.cap else max(1, self.cap // per)
+        while step > 1 and state_footprint(req.n, step, R, TB) > self.cap:
+            step -= 1
         req.chunks = [(lo, min(req.B, lo + step)) for lo in range(0, req.B, step)]
         return req
 
@@ -119,37 +210,47 @@ class ServingEngine:
             self.done.append(req)
             return
         req.t_queued = self.now()
+        R, TB = kernel_shape(req.n, req.compiled)
         for lo, hi in req.chunks:
-            R, TB = fused.interpreted_shape(req.n)
             self.queue.append(_Chunk(req, lo, hi, state_footprint(req.n, hi - lo, R, TB)))
         self._admit()
 
     def now(self) -> float:
         return time.perf_counter() - self.t0
 
+    def _order(self):
+        """Aged chunks (waited longer than max_wait_s) in arrival order, then the
+        rest by footprint (ties by arrival)."""
+        now = self.now()
+        aged = [c for c in self.queue if now - c.req.t_queued > self.max_wait_s]
+        young = [c for c in self.queue if now - c.req.t_queued <= self.max_wait_s]
+        young.sort(key=lambda c: (c.bytes, c.req.t_queued, c.lo))
+        return aged + young, len(aged)
+
     def _admit(self):
-        """First fit in arrival order: a chunk that does not fit does not block later,
-        smaller ones, unless it has waited longer than ``max_bypass_s`` (then nothing
-        passes it, so large requests cannot starve)."""
-        i = 0
-        while i < len(self.queue) and len(self.inflight) < self.max_inflight:
-            ch = self.queue[i]
-            if self.resident + ch.bytes > self.cap and self.inflight:
-                if i == 0 and ch.req.t_queued >= 0 and self.now() - ch.req.t_queued > self.max_bypass_s:
-                    break  # the head has waited long enough: hold everything for it
-                i += 1
-                continue
-            del self.queue[i]
+        """Admit in _order while memory and slots allow; the first aged chunk that
+        does not fit reserves its bytes, and later chunks only backfill the rest."""
+        while self.queue and len(self.inflight) < self.max_inflight:
+            order, naged = self._order()
+            reserve, pick = 0, None
+            for k, ch in enumerate(order):
+                if self.resident + ch.bytes + reserve <= self.cap or not self.inflight:
+                    pick = ch
+                    break
+                if reserve == 0 and k < naged:
+                    reserve = ch.bytes
+            if pick is None:
+                return
+            self.queue.remove(pick)
             try:
-                self._launch(ch)
+                self._launch(pick)
             except torch.cuda.OutOfMemoryError:
                 # the allocator could not place it after all: retry once memory frees up,
                 # or reject it when nothing else is holding memory
                 if self.inflight:
-                    self.queue.insert(i, ch)
-                    break
-                self._reject(ch.req, "out of device memory at admission")
-                i = 0  # the queue lost the request's other chunks: scan it again
+                    self.queue.append(pick)
+                    return
+                self._reject(pick.req, "out of device memory at admission")
 
     def _reject(self, req: Request, why: str):
         """Fail a request once: its queued chunks leave the queue, chunks of it still
@@ -157,7 +258,7 @@ class ServingEngine:
         if req.error is not None:
             return
         req.error = why
-        self.queue = collections.deque(c for c in self.queue if c.req is not req)
+        self.queue = [c for c in self.queue if c.req is not req]
         req.chunks = []
         req.parts = {}
         req.t_done = self.now()
@@ -175,12 +276,11 @@ class ServingEngine:
                 x = kw[prog.init[3]]
                 kw[prog.init[3]] = x[ch.lo:ch.hi] if x.dim() == 2 else x
             inp = fused.gather_inputs(prog, req.circuit.circuit, None, kw)
-            R, TB = fused.interpreted_shape(prog.n)
+            R, TB = kernel_shape(prog.n, req.compiled)
             cs = fused.build_coeffs(prog, inp)
             efac = fused.product_factors(prog, inp)
-            # interpreting kernels: a request never waits for a program to compile
-            _, probs = fused.execute(prog, inp, cs, efac, R, TB, cm=fused.INTERPRETED_COST, want_state=False,
-                                     specialise=False)
+            cm = fused.DEFAULT_COST if req.compiled else fused.INTERPRETED_COST
+            _, probs = fused.execute(prog, inp, cs, efac, R, TB, cm=cm, want_state=False, specialise=req.compiled)
             ch.out = probs
             ch.keep = (cs, efac)
             ev = torch.cuda.Event()
@@ -214,8 +314,8 @@ class ServingEngine:
             else:
                 still.append(ch)
         self.inflight = still
-        if retired:
-            self._admit()
+        if self.queue:
+            self._admit()  # also lets a chunk that just aged claim its reservation
         return retired
 
     # ----------------------------------------------------------- driver
@@ -241,12 +341,19 @@ class ServingEngine:
 def latency_stats(reqs: typing.List[Request]):
     ok = [r for r in reqs if r.error is None]
     lat = np.array([r.latency for r in ok]) * 1e3
+    svc = np.array([r.service for r in ok]) * 1e3
+
+    def pct(a, q):
+        return float(np.percentile(a, q)) if len(a) else None
+
     return {
         "requests": len(reqs),
         "rejected": len(reqs) - len(ok),
-        "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
-        "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
+        "p50_ms": pct(lat, 50),
+        "p99_ms": pct(lat, 99),
         "mean_ms": float(lat.mean()) if len(lat) else None,
+        "service_p50_ms": pct(svc, 50),
+        "service_p99_ms": pct(svc, 99),
     }
 
 

@@ -233,8 +233,11 @@ def test_28_qubits_closed_form(device):
         errs["plain per-gate"] = np.abs(circ.circuit.forward_layers(x=xd).cpu().numpy() - ref).max()
         state_circ = qandle.Circuit(body, num_qubits=n).to(device)
         psi = state_circ(x=xd)
-        norm = (psi.abs() ** 2).sum(-1).double().cpu().numpy()
-        p0 = (psi.reshape(B, 2, -1).abs() ** 2)[:, 0].sum(-1).double().cpu().numpy()  # qubit 0 = top bit
+        # float64 sums: a float32 sum of 2^28 terms drifts by about 1e-4 by itself
+        a2 = psi.abs().to(torch.float64) ** 2
+        norm = a2.sum(-1).cpu().numpy()
+        p0 = a2.reshape(B, 2, -1)[:, 0].sum(-1).cpu().numpy()  # qubit 0 is the top bit
+        del a2
         del psi
     print(f"28 qubits x 4: max |P - closed form| = {errs}; norm - 1 = {np.abs(norm - 1).max():.2e}")
     assert all(e < ATOL for e in errs.values()), errs

@@ -20,8 +20,7 @@ def _req(rid, n, B, depth, seed, device="cpu", arrival=0.0):
 
 
 def test_admission_footprint_chunking_and_rejection():
-    eng = serving.ServingEngine.__new__(serving.ServingEngine)  # host-side parts only
-    eng.cap = 10 * (2 ** 12) * 8 + 50_000
+    eng = _host_engine(10 * (2 ** 12) * 8 + 50_000)  # host-side parts only
     r, _ = _req(0, 12, 37, 2, 1)
     eng.prepare(r)
     assert r.error is None and r.n == 12 and r.B == 37
@@ -53,14 +52,20 @@ def test_results_identical_alone_and_in_mix(device):
         ws.append(w)
         r2, _ = _req(i, n, B, d, 100 + i, device, arrival=0.0)
         reqs_alone.append(r2)
-    eng = serving.ServingEngine(device, cap_bytes=64 * 2 ** 20, streams=4)
+    # requests of 12 qubits and more run compiled kernels, built ahead (in this process: a
+    # process that already uses the GPU starts no compile workers)
+    nstruct = serving.precompile(reqs_mix, 12, workers=1)
+    assert nstruct == len({(n, d) for n, _, d in spec if n >= 12})
+    eng = serving.ServingEngine(device, cap_bytes=64 * 2 ** 20, streams=4, compiled_from_qubits=12)
     for r in reqs_mix:
         eng.prepare(r)
+    assert {r.compiled for r in reqs_mix} == {True, False}
     done = eng.serve(reqs_mix)
     assert len(done) == len(spec) and all(r.error is None for r in done)
     assert eng.peak_resident <= 64 * 2 ** 20
     for r in reqs_alone:
-        solo = serving.ServingEngine(device, cap_bytes=64 * 2 ** 20, streams=1, max_inflight=1)
+        solo = serving.ServingEngine(device, cap_bytes=64 * 2 ** 20, streams=1, max_inflight=1,
+                                     compiled_from_qubits=12)
         solo.prepare(r)
         solo.serve([r])
     for a, b, w in zip(reqs_mix, reqs_alone, ws):
@@ -71,6 +76,7 @@ def test_results_identical_alone_and_in_mix(device):
         assert np.abs(reqs_mix[i].result.cpu().numpy() - ref).max() < 1e-5
     st = serving.latency_stats(done)
     assert st["p50_ms"] > 0 and st["p99_ms"] >= st["p50_ms"]
+    assert 0 < st["service_p50_ms"] <= st["p99_ms"]
 
 
 class _DoneEvent:
@@ -80,12 +86,11 @@ class _DoneEvent:
 
 def _host_engine(cap):
     """A ServingEngine with its host-side state only (no streams, no device)."""
-    import collections
     import time
 
     eng = serving.ServingEngine.__new__(serving.ServingEngine)
-    eng.cap, eng.max_inflight, eng.max_bypass_s = int(cap), 64, 2.0
-    eng.queue, eng.inflight, eng.done = collections.deque(), [], []
+    eng.cap, eng.max_inflight, eng.max_wait_s, eng.compiled_from = int(cap), 64, 2.0, None
+    eng.queue, eng.inflight, eng.done = [], [], []
     eng.resident = eng.peak_resident = eng._rr = 0
     eng.t0 = time.perf_counter()
     return eng
@@ -133,3 +138,66 @@ def test_chunk_of_a_rejected_request_is_dropped_when_it_retires():
     eng._reject(r, "a second failure is not reported twice")
     eng.poll()
     assert eng.done == [r] and r.result is None and eng.resident == 0 and not eng.inflight
+
+
+def test_admission_order_shortest_first_aging_and_backfill(monkeypatch):
+    """Young chunks go smallest first; a chunk that waited past max_wait_s goes
+    first, and when it does not fit, only chunks that leave room for it are
+    admitted until it does."""
+    eng = _host_engine(1000)
+    launched = []
+
+    def fake_launch(ch):
+        launched.append(ch.req.rid)
+        ch.event, ch.out = _Pending(), torch.zeros(ch.hi - ch.lo, 1)
+        eng.resident += ch.bytes
+        eng.inflight.append(ch)
+
+    monkeypatch.setattr(eng, "_launch", fake_launch)
+
+    def req(rid, nbytes, queued):
+        r = serving.Request(rid, None, {})
+        r.n, r.B, r.t_queued, r.chunks = 1, 1, queued, [(0, 1)]
+        return serving._Chunk(r, 0, 1, nbytes)
+
+    now = eng.now()
+    # 600 B already resident; queue: a young 300, a young 100, an aged 500 (waited 5 s)
+    eng.resident = 600
+    eng.inflight.append(serving._Chunk(serving.Request(99, None, {}), 0, 1, 600, event=_Pending()))
+    eng.queue = [req(1, 300, now), req(2, 100, now), req(3, 500, now - 5.0)]
+    eng._admit()
+    # the aged 500 does not fit (600 + 500 > 1000) and reserves 500: only chunks that keep
+    # resident + bytes <= 500 may pass, and 600 is already resident, so nothing is admitted
+    assert launched == []
+    # the resident work retires: the aged chunk goes first, then the smallest young ones that fit
+    eng.inflight, eng.resident = [], 0
+    eng._admit()
+    assert launched == [3, 2, 1][: len(launched)] and launched[0] == 3 and launched[1] == 2
+    assert eng.resident <= 1000
+
+
+class _Pending:
+    def query(self):
+        return False
+
+
+def test_precompile_workers_fill_the_code_cache(tmp_path, monkeypatch):
+    """precompile in worker processes (hipRTC only, nothing loaded onto a device)
+    writes one code object per program structure; building the same structures
+    again in this process then finds every source in the cache."""
+    import os
+
+    from qandle_amd._engine import jit
+
+    cache = str(tmp_path / "jit")
+    monkeypatch.setenv("QANDLE_AMD_JIT_CACHE", cache)  # read by the spawned workers at import
+    monkeypatch.setattr(jit, "CACHE_DIR", cache)
+    reqs = [_req(i, n, 2, d, 40 + i)[0] for i, (n, d) in enumerate([(13, 2), (14, 3), (13, 2), (9, 2)])]
+    assert serving.precompile(reqs, 12, workers=2) == 2   # (13, 2) twice; 9 qubits is below the threshold
+    assert len([f for f in os.listdir(cache) if f.endswith(".hsaco")]) == 2
+
+    def no_compile(*a, **k):
+        raise AssertionError("source not found in the code-object cache")
+
+    monkeypatch.setattr(jit, "_compile", no_compile)
+    assert serving.precompile(reqs, 12, workers=1) == 2

@@ -162,3 +162,90 @@ def test_hermitian_trace_matches_the_full_trace():
             full = np.trace(Q @ M).imag
             herm = Q[0, 0].real * h0 + Q[1, 1].real * h1 + Q[1, 0].real * h2 + Q[1, 0].imag * h3
             assert abs(full - herm) < 1e-9 * max(1.0, abs(full))
+
+
+@pytest.mark.parametrize("R,TB,lpos", [(5, 8, list(range(13))), (4, 8, [0, 1, 2, 3, 7, 8, 9, 10, 14, 15, 16, 18]),
+                                       (5, 6, [0, 1, 2, 3, 5, 6, 9, 12, 13, 17, 19])])
+def test_async_tile_loads_cover_the_tile_once(R, TB, lpos):
+    """Every 16-byte slot of a buffer gets the amplitude pair the kernel's swizzled
+    layout puts there: slot s = ii * 2^TB + tid holds tile elements swz_e(2s) and
+    swz_e(2s) + 1, loaded from amplitude offset scatter(swz_e(2s)); over all loads of
+    all threads each tile amplitude is loaded exactly once, into its own slot."""
+    from qandle_amd._engine import jit
+
+    K, T = R + TB, 1 << TB
+    d_bit, d_inst = jit.async_load_offsets(lpos, R, TB)
+    scat = lambda l: sum(1 << lpos[k] for k in range(K) if (l >> k) & 1)  # noqa: E731
+    seen = set()
+    for ii in range(1 << (R - 1)):
+        for tid in range(T):
+            off = d_inst[ii]
+            for b in range(TB):
+                if (tid >> b) & 1:
+                    off ^= d_bit[b]
+            s = ii * T + tid
+            el = PL.swz(2 * s, 30)
+            assert el % 2 == 0 and PL.swz(el, 30) == 2 * s       # the pair stays in order; swz is an involution
+            assert off == scat(el) and off % 2 == 0               # 16-byte aligned pair start
+            seen.add(off)
+            seen.add(off + 1)
+    assert seen == {scat(l) for l in range(1 << K)}
+
+
+def test_async_pass_sources_compile(tmp_path):
+    """The asynchronous-load pass kernels of a 14-qubit program compile (hipRTC, no
+    device), and in the code object the loop's arithmetic starts while the next
+    tile's loads are in flight: after each block of prefetch loads, a matrix-core
+    instruction comes before any wait on the vector-memory counter."""
+    import re
+    import subprocess
+    from qandle_amd import workloads
+    from qandle_amd._engine import fused, jit
+
+    w = workloads.rxryrz_ladder(n=14, B=2, depth=3, seed=4)
+    circ = qandle.Circuit(w.layers, num_qubits=14)
+    prog = fused.lower_circuit(circ.circuit)
+    lead = fused.leading_rotations(prog)
+    p = lead[0] if lead else prog
+    pl = fused.get_plan(p, 5, 8, None, True)
+    ws = fused.pass_words(p, pl, 2, "cpu")
+    masks = fused.readout_fold(p)[1]
+    off = 0
+    srcs = []
+    for i, (wd, _ns, lpos) in enumerate(ws.passes):
+        words = wd.numpy()
+        if i > 0:
+            srcs.append(jit.pass_source_async(f"qs_pass_{i}", words, 14, 5, 8, lpos, i == len(ws.passes) - 1, 2,
+                                              masks, "lds", off, 4, True))
+        off += len(jit.lane_entries(words, 5, 8, False))
+    assert srcs
+    code = jit._compile(jit.PRELUDE + "\n\n".join(srcs), "async_probe")
+    assert len(code) > 0
+    objdump = "/opt/rocm/llvm/bin/llvm-objdump"
+    if not os.path.exists(objdump):
+        pytest.skip("llvm-objdump not installed")
+    path = tmp_path / "async.hsaco"
+    path.write_bytes(code)
+    asm = subprocess.run([objdump, "-d", "--mcpu=gfx950", str(path)], capture_output=True, text=True).stdout
+    ops = [m.group(1) for m in re.finditer(r"^\s+([a-z_0-9]+)", asm, re.M)]
+    waits = [i for i, line in enumerate(asm.splitlines()) if "s_waitcnt" in line and "vmcnt" in line]
+    lines = asm.splitlines()
+    blocks = 0
+    i = 0
+    while i < len(lines):
+        if "global_load_lds_dwordx4" in lines[i]:
+            j = i
+            while j < len(lines) and "global_load_lds_dwordx4" in lines[j]:
+                j += 1
+            # the first block of a kernel is its prologue load: nothing to overlap yet
+            prev = "\n".join(lines[max(0, i - 40):i])
+            if "s_barrier" in prev:
+                nxt_wait = next((k for k in range(j, len(lines)) if "s_waitcnt" in lines[k] and "vmcnt" in lines[k]),
+                                len(lines))
+                nxt_mfma = next((k for k in range(j, len(lines)) if "v_mfma" in lines[k]), len(lines))
+                assert nxt_mfma < nxt_wait, f"a vmcnt wait right after the prefetch at line {i}"
+                blocks += 1
+            i = j
+        else:
+            i += 1
+    assert ops and waits and blocks >= 2
