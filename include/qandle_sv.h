@@ -14,7 +14,7 @@
  *     qs_last_hip_error() gives the HIP error behind QS_EHIP.
  *   - No entry point allocates device memory.  Entry points that need scratch
  *     take a caller-provided `work` buffer whose size is given by the matching
- *     *_work_bytes query.
+ *     *_work_bytes query (qs_measure_prob_noalloc uses a static device array).
  *
  * Reference citations are file:line into gstenzel/qandle v0.1.8.
  */
@@ -81,6 +81,19 @@ int qs_cnot(float *out, const float *in, int64_t B, int n, int cbit, int tbit,
 int64_t qs_measure_prob_work_bytes(int64_t B, int n);
 int qs_measure_prob(float *out, const float *psi, int64_t B, int n, void *work,
                     int64_t work_bytes, void *stream);
+
+/* qs_measure_prob without a workspace argument, for callers that cannot
+ * allocate (code under HIP graph capture, serving loops and harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The stage-1 partials
+ * live in a file-scope device array of QS_NOALLOC_SCRATCH_BYTES, one per
+ * device for the life of the process; the call allocates nothing.  Requests
+ * whose partials exceed it (qs_measure_prob_work_bytes(B, n) > 16 MiB, i.e.
+ * more than about 2^30 amplitudes) return QS_EWORKSPACE.  Every call on one
+ * device shares that array: calls that overlap on one device (other streams,
+ * other host threads) race and must be ordered by the caller. */
+#define QS_NOALLOC_SCRATCH_BYTES (16 << 20)
+int qs_measure_prob_noalloc(float *out, const float *psi, int64_t B, int n,
+                            void *stream);
 
 /* Backward of MeasureProbability: lam[b, i] = 2 psi[b, i] * sum_q w[b, q] [bit n-1-q of i = 0]
  * (torch's dL/dRe + i dL/dIm for L = sum w * P).  w is (B, n) float32 with

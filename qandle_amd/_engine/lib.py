@@ -35,6 +35,7 @@ SIGNATURES = {
     "qs_cnot": (I32, [P, P, I64, I32, I32, I32, P]),
     "qs_measure_prob_work_bytes": (I64, [I64, I32]),
     "qs_measure_prob": (I32, [P, P, I64, I32, P, I64, P]),
+    "qs_measure_prob_noalloc": (I32, [P, P, I64, I32, P]),
     "qs_measure_prob_backward": (I32, [P, P, P, I64, I64, I32, P]),
     "qs_pauli_im_work_bytes": (I64, [I64, I32, I32]),
     "qs_pauli_im": (I32, [P, P, P, I64, I32, P, I32, I32, P, I64, P]),

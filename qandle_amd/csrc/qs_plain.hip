@@ -4,6 +4,8 @@
 // its own (BuiltRX.forward, BuiltCNOT.forward, ...).  The fused program path
 // (qs_fused.hip) is what Circuit.forward uses; this file is also its
 // correctness and speed comparator.
+#include <mutex>
+
 #include "qs_common.h"
 
 namespace qs {
@@ -410,21 +412,79 @@ int64_t qs_measure_prob_work_bytes(int64_t B, int n) {
   return (B << (n - c)) * (c + 1) * (int64_t)sizeof(float);
 }
 
+}  // extern "C"
+
+// The two MeasureProbability stages over a partials buffer of
+// qs_measure_prob_work_bytes(B, n) bytes (shapes checked by the caller).
+static int measure_launch(float *out, const float *psi, int64_t B, int n, float *part, hipStream_t st) {
+  const int c = chunk_bits(n);
+  const int64_t nch = B << (n - c);
+  if (nch > 0x7fffffff || B > 0x7fffffff) return QS_EUNSUPPORTED;
+  hipLaunchKernelGGL(k_measure_stage1, dim3((unsigned)nch), dim3(kThreads), 0, st, part, psi, c);
+  int s = launch_status();
+  if (s) return s;
+  hipLaunchKernelGGL(k_measure_stage2, dim3((unsigned)B, (unsigned)n), dim3(kThreads), 0, st, out,
+                     (const float *)part, n, c);
+  return launch_status();
+}
+
+// Stage-1 partials of qs_measure_prob_noalloc.  A file-scope device array: every
+// device holds its own copy, at its own address, for the life of the process.
+// Host code never passes `scratch` itself to a kernel (on the host that name is
+// a stand-in, not a device address); it uses the address looked up per device
+// below.  Stage 1 writes every partial that stage 2 reads, so nothing needs
+// clearing between calls.
+static __device__ __align__(256) float scratch[QS_NOALLOC_SCRATCH_BYTES / sizeof(float)];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[64];  // device address of `scratch` per device; null until looked up
+
+// Address of `scratch` on the device of `stream` (the current device for the
+// null stream), looked up once per device with that device current.
+static int scratch_for(hipStream_t stream, float **out) {
+  int dev = 0;
+  hipError_t e = stream ? hipStreamGetDevice(stream, &dev) : hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    g_last_hip_error = (int)e;
+    return QS_EHIP;
+  }
+  if (dev < 0 || dev >= 64) return QS_EUNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = 0;
+    if ((e = hipGetDevice(&cur)) != hipSuccess || (cur != dev && (e = hipSetDevice(dev)) != hipSuccess)) {
+      g_last_hip_error = (int)e;
+      return QS_EHIP;
+    }
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p) {
+      g_last_hip_error = (int)e;
+      return QS_EHIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = static_cast<float *>(g_scratch_ptr[dev]);
+  return QS_OK;
+}
+
+extern "C" {
+
 int qs_measure_prob(float *out, const float *psi, int64_t B, int n, void *work, int64_t work_bytes,
                     void *stream) {
   if (bad_shape(B, n) || !out || !psi || !work) return QS_EINVAL;
   if (work_bytes < qs_measure_prob_work_bytes(B, n)) return QS_EWORKSPACE;
-  const int c = chunk_bits(n);
-  const int64_t nch = B << (n - c);
-  if (nch > 0x7fffffff || B > 0x7fffffff) return QS_EUNSUPPORTED;
-  hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_measure_stage1, dim3((unsigned)nch), dim3(kThreads), 0, st, (float *)work,
-                     psi, c);
-  int s = launch_status();
+  return measure_launch(out, psi, B, n, (float *)work, (hipStream_t)stream);
+}
+
+int qs_measure_prob_noalloc(float *out, const float *psi, int64_t B, int n, void *stream) {
+  if (bad_shape(B, n) || !out || !psi) return QS_EINVAL;
+  if (qs_measure_prob_work_bytes(B, n) > (int64_t)QS_NOALLOC_SCRATCH_BYTES) return QS_EWORKSPACE;
+  float *part = nullptr;
+  const int s = scratch_for((hipStream_t)stream, &part);
   if (s) return s;
-  hipLaunchKernelGGL(k_measure_stage2, dim3((unsigned)B, (unsigned)n), dim3(kThreads), 0, st, out,
-                     (const float *)work, n, c);
-  return launch_status();
+  return measure_launch(out, psi, B, n, part, (hipStream_t)stream);
 }
 
 int qs_measure_prob_backward(float *lam, const float *psi, const float *w, int64_t w_stride,
