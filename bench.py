@@ -158,7 +158,12 @@ def profile_workload(mode, reps=4):
 def summarize(db: str, last: int):
     """Kernel table (calls, total, average, min, max in microseconds) of a rocprofv3
     results database; with ``last`` > 0 only the last ``last`` dispatches of every
-    kernel name count (the steady-state steps after compiles and settling)."""
+    kernel name count (the steady-state steps after compiles and settling).  A
+    ``*counter_collection.csv`` file instead gives, per kernel and counter, the
+    average value per dispatch over the same last dispatches."""
+    if db.endswith(".csv"):
+        summarize_counters(db, last)
+        return
     import sqlite3
 
     con = sqlite3.connect(db)
@@ -176,6 +181,26 @@ def summarize(db: str, last: int):
     for tot, name, ds in sorted(out, reverse=True):
         short = name if len(name) < 70 else name[:67] + "..."
         print(f"| `{short}` | {len(ds)} | {tot:.1f} | {tot / len(ds):.1f} | {min(ds):.1f} | {max(ds):.1f} |")
+
+
+def summarize_counters(path: str, last: int):
+    import csv
+
+    per = {}  # (kernel, counter) -> {dispatch: value}
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            key = (row["Kernel_Name"], row["Counter_Name"])
+            d = per.setdefault(key, {})
+            did = int(row["Dispatch_Id"])
+            d[did] = d.get(did, 0.0) + float(row["Counter_Value"])
+    print("| Kernel | Counter | Dispatches | Average per dispatch |")
+    print("|---|---|---|---|")
+    for (kern, ctr), d in sorted(per.items()):
+        vals = [d[k] for k in sorted(d)]
+        if last > 0:
+            vals = vals[-last:]
+        short = kern if len(kern) < 70 else kern[:67] + "..."
+        print(f"| `{short}` | {ctr} | {len(vals)} | {sum(vals) / len(vals):.6g} |")
 
 
 # ----------------------------------------------------------------------------

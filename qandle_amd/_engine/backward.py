@@ -155,28 +155,22 @@ class FusedCircuitFn(torch.autograd.Function):
         L.check(lib.qs_adjoint_finish(L.ptr(per), L.ptr(tot), L.ptr(gpart), total_rows, B, n, R, TB, st),
                 "qs_adjoint_finish")
 
+        scatter = gradient_scatter(prog, rows, psi.device)
         # parameters (shared angles): one row each
         g_param = None
         if ctx.needs_input_grad[7]:
-            param_rows = []
-            slot_pos = {s_: i for i, s_ in enumerate(prog.param_slots)}
-            for gi, g in enumerate(prog.gates):
-                if g.kind != 4 and prog.slots[g.slot].kind == "param":
-                    param_rows.append((slot_pos[g.slot], rows[gi]))
             g_param = torch.zeros(len(prog.param_slots), dtype=torch.float32, device=psi.device)
-            if param_rows:
-                idx = torch.tensor([p for p, _ in param_rows], device=psi.device)
-                src = tot[torch.tensor([r for _, r in param_rows], device=psi.device)]
-                g_param.index_add_(0, idx, src)
+            if scatter["param"] is not None:
+                idx, src_rows = scatter["param"]
+                g_param.index_add_(0, idx, tot[src_rows])
         # named inputs: per-sample rows summed over the gates that use the name
         g_named = []
         for k, name in enumerate(ctx.names):
             if not ctx.needs_input_grad[10 + k]:
                 g_named.append(None)
                 continue
-            rr = [rows[gi] for gi, g in enumerate(prog.gates) if g.kind != 4 and prog.slots[g.slot].kind == "named"
-                  and prog.slots[g.slot].name == name]
-            gv = per[torch.tensor(rr, device=psi.device)].sum(0) if rr else torch.zeros(B, device=psi.device)
+            rr = scatter["named"].get(name)
+            gv = per[rr].sum(0) if rr is not None else torch.zeros(B, device=psi.device)
             shape = ctx.named_shapes[k]
             g_named.append(gv.sum().reshape(shape) if len(shape) == 0 else gv.reshape(shape))
         g_embed = None
@@ -189,6 +183,32 @@ class FusedCircuitFn(torch.autograd.Function):
             shp = ctx.state_shape
             g_state = lam.sum(0) if len(shp) == 1 else (lam if lam.shape == shp else lam.sum(0, keepdim=True))
         return (None, None, None, None, None, None, None, g_param, g_embed, g_state, *g_named)
+
+
+def gradient_scatter(prog, rows, device):
+    """Index tensors that route gradient rows to angles, uploaded once per program
+    and device: ``param`` = (angle index, row) of the unnamed rotations (None
+    when there are none), ``named`` = rows of each named input's gates."""
+    key = ("grad_scatter", str(device))
+    hit = prog.words_cache.get(key)
+    if hit is None:
+        slot_pos = {s_: i for i, s_ in enumerate(prog.param_slots)}
+        param_rows, named = [], {}
+        for gi, g in enumerate(prog.gates):
+            if g.kind == 4:  # CNOT
+                continue
+            sl = prog.slots[g.slot]
+            if sl.kind == "param":
+                param_rows.append((slot_pos[g.slot], rows[gi]))
+            elif sl.kind == "named":
+                named.setdefault(sl.name, []).append(rows[gi])
+        hit = {
+            "param": (torch.tensor([p for p, _ in param_rows], device=device),
+                      torch.tensor([r for _, r in param_rows], device=device)) if param_rows else None,
+            "named": {nm: torch.tensor(rr, device=device) for nm, rr in named.items()},
+        }
+        prog.words_cache[key] = hit
+    return hit
 
 
 def adjoint_geometry(B: int, n: int, R: int, TB: int):

@@ -702,10 +702,17 @@ def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu
         for r, ui, ps in bits:
             off = f"8 * ({ui} + csample)" if ps else f"{8 * ui}"
             e("      {")
-            if (si_, r) in cq and not valu_pairs:
+            if (si_, r) in cq:
                 nm = cq[(si_, r)]
                 e(f"        const float m[4] = {{{nm}.x, {nm}.y, {nm}.z, {nm}.w}};")
                 prs = [(phys[j], phys[j | (1 << r)]) for j in range(NR) if not (j >> r) & 1]
+                # the last `valu_pairs` pairs run on the vector ALU, spread between the
+                # matrix-core groups so that both units have work at the same time
+                nv = min(int(valu_pairs), len(prs))
+                vprs = prs[len(prs) - nv:]
+                prs = prs[:len(prs) - nv]
+                if vprs:
+                    e("        const int *ut = pin(A.utab);")
                 step = 4 if interleave == 4 else (2 if interleave else 1)
                 q = 0
                 while q < len(prs):
@@ -719,6 +726,11 @@ def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu
                     else:
                         e(f"        mfma_pair(v[{grp[0][0]}], v[{grp[0][1]}], m);")
                     q += len(grp)
+                    if vprs:
+                        a, b = vprs.pop(0)
+                        e(f"        valu_pair_s(v[{a}], v[{b}], ut + {off});")
+                for a, b in vprs:
+                    e(f"        valu_pair_s(v[{a}], v[{b}], ut + {off});")
             elif ps and multi:
                 e("        const int *ut = pin(A.utab);")
                 e(f"        const int *u = ut + {off};")
