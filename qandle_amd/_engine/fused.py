@@ -425,6 +425,46 @@ def _first_angle(theta: torch.Tensor) -> torch.Tensor:
     return theta.reshape(1) if theta.numel() == 1 else theta.reshape(-1)[:1]
 
 
+def _gather_thetas(thetas, cache=None) -> torch.Tensor:
+    """One (len,) tensor of the rotations' angles (the first element of each theta),
+    in order, with as few autograd nodes as the shapes allow.
+
+    A circuit holds one theta Parameter per gate (the reference's state_dict
+    layout), so a deep circuit has hundreds of them; one reshape per theta costs
+    more host time than the circuit's kernels.  0-d thetas (angles given as
+    numbers, ansatz parameters) go through one ``stack``, one-element 1-d thetas
+    (default ``torch.rand(1)`` draws) through one ``cat``, anything else through a
+    reshape; mixed shapes are put back in order with one gather."""
+    zero = [i for i, t in enumerate(thetas) if t.dim() == 0]
+    if len(zero) == len(thetas):
+        return torch.stack(thetas)
+    one = [i for i, t in enumerate(thetas) if t.dim() == 1 and t.shape[0] == 1]
+    if len(one) == len(thetas):
+        return torch.cat(thetas)
+    rest = [i for i in range(len(thetas)) if i not in set(zero) | set(one)]
+    parts, order = [], []
+    if zero:
+        parts.append(torch.stack([thetas[i] for i in zero]))
+        order += zero
+    if one:
+        parts.append(torch.cat([thetas[i] for i in one]))
+        order += one
+    if rest:
+        parts.append(torch.cat([_first_angle(thetas[i]) for i in rest]))
+        order += rest
+    flat = torch.cat(parts)
+    key = ("theta_order", str(flat.device), tuple(order))
+    idx = None if cache is None else cache.get(key)
+    if idx is None:
+        pos = [0] * len(order)
+        for j, i in enumerate(order):
+            pos[i] = j
+        idx = torch.tensor(pos, dtype=torch.long, device=flat.device)
+        if cache is not None:
+            cache[key] = idx
+    return flat.index_select(0, idx)
+
+
 def param_angles(prog: Program, device) -> torch.Tensor:
     """Remapped angles of the program's unnamed rotations, in slot order.
 
@@ -452,9 +492,9 @@ def param_angles(prog: Program, device) -> torch.Tensor:
         else:
             values.append(k)
     for fn, ks in groups.items():
-        # a one-element theta enters through a reshape view (its gradient comes
-        # back as a slice of one buffer, with no per-parameter kernel)
-        th = torch.cat([_first_angle(params[k].module.theta) for k in ks])
+        # one stack/cat of the thetas: their gradients come back as views of one buffer,
+        # with no per-parameter kernel
+        th = _gather_thetas([params[k].module.theta for k in ks], prog.words_cache)
         pieces.append(fn(th).to(device=device, dtype=torch.float32))
         order.append(ks)
     if values:
