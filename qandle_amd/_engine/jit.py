@@ -511,6 +511,95 @@ def pass_source_async(name: str, words, n: int, R: int, TB: int, lpos, measure: 
     return "\n".join(o)
 
 
+def pass_source_tail(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, measure: bool, waves: int = 2,
+                     readout=None, lanes="lds", lane_off: int = 0, interleave: int = 4,
+                     wave_sync: bool = True) -> str:
+    """HIP source of one specialised forward pass kernel that streams the next
+    tile in during the last sub-pass of the current one.  A resident grid (two
+    workgroups per compute unit, one tile buffer each, as pass_source) walks
+    tiles; the last sub-pass stores its results from registers straight to the
+    state (coalesced runs: the lane bits are the lowest non-register tile bits),
+    so once its threads hold their amplitudes the tile buffer is free, and the
+    next tile's direct global->LDS loads (``global_load_lds_dwordx4``) run
+    while its arithmetic does.  The tile layout keeps amplitude pairs in order
+    in one 16-byte slot (``swz_e``).  Tiles within one sample only (n >= R + TB).
+    Same sub-pass code, tables and partial layouts as pass_source: results are
+    the same bit for bit (measured at 14, 16, 20 and 24 qubits).  Measured no
+    faster: 2.38 vs 2.32 ms per 20-qubit batch-64 forward, 3.39 vs 3.14 ms at
+    24 qubits x 4."""
+    K = R + TB
+    T, NR = 1 << TB, 1 << R
+    if n < K:
+        raise ValueError("tile streaming needs tiles within one sample")
+    subs = PL.decode_pass(words, R, TB)
+    lpos = [int(x) for x in lpos]
+    npos = [p for p in range(n) if p not in lpos]
+    nn = len(npos)
+    o = []
+    e = o.append
+    wattr = f" __attribute__((amdgpu_waves_per_eu({waves})))" if waves else ""
+    e(f'extern "C" __global__ void __launch_bounds__({T}){wattr}')
+    e(f"{name}(JArgs A) {{")
+    e(f"  __shared__ __align__(16) float2 tile[{1 << K}];")
+    if measure and TB > 6:
+        e(f"  __shared__ float red[{(T // 64) * ((K + 1) if readout is None else (n + 1))}];")
+    lane_k = None
+    if lanes == "lds":
+        ents = lane_entries(words, R, TB, False)
+        lane_k = {ui: k for k, ui in enumerate(ents)}
+        if ents:
+            e(f"  __shared__ __align__(16) float4 ctab[{4 * len(ents)}];")
+    e("  const int tid = threadIdx.x;")
+    e("  auto tile_base = [&](i64 t) -> i64 {")
+    e(f"    i64 b = (t >> {nn}) << {n};")
+    for i, p in enumerate(npos):
+        e(f"    b |= ((t >> {i}) & 1) << {p};")
+    e("    return b;")
+    e("  };")
+    if not gen:
+        d_bit, d_inst = async_load_offsets(lpos, R, TB)
+        e("  i64 d_tid = 0;")
+        for b in range(TB):
+            e(f"  d_tid ^= ((tid >> {b}) & 1) ? (i64){d_bit[b]}LL : 0LL;")
+        e("  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);")
+        e("  auto issue = [&](i64 tt) {")
+        e("    const i64 b = tile_base(tt);")
+        e("    const float4 *src = reinterpret_cast<const float4 *>(A.src);")
+        for ii in range(NR // 2):
+            e(f"    __builtin_amdgcn_global_load_lds(src + ((b | (d_tid ^ {d_inst[ii]}LL)) >> 1),")
+            e(f"        (__attribute__((address_space(3))) void *)(tile + 2 * ({ii * T} + 64 * wave)), 16, 0, 0);")
+        e("  };")
+    if lane_k:
+        nl = 4 * len(lane_k)
+        e(f"  for (int c = tid; c < {nl}; c += {T}) ctab[c] = reinterpret_cast<const float4 *>(A.ltab)[{4 * lane_off} + c];")
+    e("  const i64 G = gridDim.x;")
+    e("  i64 t = blockIdx.x;")
+    if not gen:
+        e("  if (t < A.ntiles) issue(t);")
+    else:
+        e("  __syncthreads();")
+    e("  for (; t < A.ntiles; t += G) {")
+    if not gen:
+        # s_waitcnt vmcnt(0) through the builtin (the compiler's wait tracking sees it): this wave's
+        # loads of the tile, and its earlier stores, have completed; then every wave's
+        e("    __builtin_amdgcn_s_waitcnt(0x0F70); __syncthreads();")
+    e("    const i64 base = tile_base(t);")
+    e(f"    const i64 block_sample = base >> {n};")
+    e(f"    float2 v[{NR}];")
+    prefetch = [] if gen else ["      if (t + G < A.ntiles) issue(t + G);"]
+    for si, sub in enumerate(subs):
+        last = si == len(subs) - 1
+        _sub_pass(e, sub, si, len(subs), n, R, TB, False, gen and si == 0, measure and last, readout, 0,
+                  lanes=lane_k if lanes == "lds" else lanes, interleave=interleave,
+                  wave_sync=wave_sync and not last and wave_local(sub, subs[si + 1], TB), swz_mask=30,
+                  tail=prefetch if last else None)
+    if gen:
+        e("    __syncthreads();")  # the next tile's first sub-pass writes the tile buffer again
+    e("  }")
+    e("}")
+    return "\n".join(o)
+
+
 def pass_source_pipelined(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, measure: bool,
                           waves: int = None, readout=None, loaders: int = 8) -> str:
     """HIP source of one specialised forward pass kernel with a load/compute
@@ -618,11 +707,14 @@ def wave_local(a, b, TB: int) -> bool:
 
 
 def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu_pairs=0, pipe=None, lanes=False,
-              interleave=False, wave_sync=False, swz_mask=31):
+              interleave=False, wave_sync=False, swz_mask=31, tail=None):
     """One sub-pass.  ``pipe`` (pipelined kernels): a dict; the tile is
     ``buf[boff + i]``, no closing barrier is emitted, and a measurement's
     cross-wave finish is left in ``pipe["finish"]`` for after the caller's
-    barrier."""
+    barrier.  ``tail`` (the last sub-pass of pass_source_tail): a list of lines
+    emitted after a block barrier that follows the register loads (the tile
+    buffer is then free), and the results go straight from registers to the
+    state instead of back into the tile."""
     if pipe is not None:
         import re
 
@@ -697,6 +789,10 @@ def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu
                 e(f"      v[{j}] = lds_at(tile, lb8 ^ {8 * swj[j]});")
             else:
                 e(f"      v[{j}] = tile[lb ^ {swj[j]}];")
+    if tail is not None:
+        e("      __syncthreads();  // every thread holds its amplitudes: the tile buffer is free")
+        for line in tail:
+            e(line)
     phys = list(range(NR))
     for si_, (bits, cnots) in enumerate(sub.segments):
         for r, ui, ps in bits:
@@ -777,6 +873,12 @@ def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu
         for j in range(NR):
             e(f"        v[{j}] = cmul(z, v[{j}]);")
         e("      }")
+    if tail is not None:
+        _direct_store(e, sub, phys, R)
+        if measure:
+            _measure(e, sub, phys, n, R, TB, multi, readout, pipe)
+        e("    }")
+        return
     for j in range(NR):
         if pipe is None:
             e(f"      lds_at(tile, lb8 ^ {8 * swj[j]}) = v[{phys[j]}];")
@@ -792,6 +894,29 @@ def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu
     elif pipe is None:
         e("      __syncthreads();")
     e("    }")
+
+
+def _direct_store(e, sub, phys, R):
+    """Store a sub-pass's registers straight to the state: register j of a thread
+    is amplitude gbase | sum of 1 << GP[r] over its set bits r.  The lane bits hold
+    the lowest non-register tile bits in ascending order (planner.thread_bits), so
+    a wave's stores form contiguous runs; when a register bit sits at amplitude
+    bit 0, two registers make one 16-byte pair store."""
+    NR = 1 << R
+    roff = [sum(1 << sub.GP[r] for r in range(R) if (j >> r) & 1) for j in range(NR)]
+    r0 = next((r for r in range(R) if sub.GP[r] == 0), None)
+    e("      if (A.store) {")
+    e("        float4 *o4 = reinterpret_cast<float4 *>(A.state);")
+    e("        float2 *o2 = reinterpret_cast<float2 *>(A.state);")
+    for j in range(NR):
+        if r0 is not None:
+            if (j >> r0) & 1:
+                continue
+            a, b = phys[j], phys[j | (1 << r0)]
+            e(f"        o4[(gbase | {roff[j]}LL) >> 1] = make_float4(v[{a}].x, v[{a}].y, v[{b}].x, v[{b}].y);")
+        else:
+            e(f"        o2[gbase | {roff[j]}LL] = v[{phys[j]}];")
+    e("      }")
 
 
 def _parity(var: str, mask: int) -> str:
@@ -1311,13 +1436,16 @@ class ProgramKernels:
     vs 3.67 ms per 20-qubit forward, 4.69 vs 4.89 at 24 qubits, results bit
     for bit the same.  ``interleave`` (pass_source) = 4 is a default too:
     within 1% of 2 and of none on these shapes, and each step halves the
-    MFMA wait states."""
+    MFMA wait states.  ``async_loads`` (pass_source_async: two tile buffers,
+    one workgroup per compute unit) and ``tail_prefetch`` (pass_source_tail:
+    the next tile streams in during the last sub-pass) are options that give
+    the same bits and measured slower or no faster."""
 
     def __init__(self, passes, n: int, R: int, TB: int, gen_first: bool, measure: bool,
                  prefetch: bool = False, waves: int = 2, readout=None, valu_pairs: int = 0,
                  grid: str = "full", pipeline: int = 0, lanes="lds", split_pairs: bool = True,
                  interleave: int = 4, wave_sync: bool = True, stagger: int = 0, load: bool = True,
-                 async_loads: bool = False):
+                 async_loads: bool = False, tail_prefetch: bool = False):
         self.n, self.R, self.TB = n, R, TB
         self.pipelined = bool(pipeline) and n >= R + TB
         self.block = (1 << TB) + ((1 << pipeline) if self.pipelined else 0)
@@ -1333,8 +1461,13 @@ class ProgramKernels:
             if lanes == "lds" and not self.pipelined:
                 self.lane_index += lane_entries(words, R, TB, n < R + TB)
             use_async = async_loads and not self.pipelined and n >= R + TB and not (gen_first and i == 0)
-            self.persistent.append(use_async)
-            if use_async:
+            use_tail = tail_prefetch and not use_async and not self.pipelined and n >= R + TB
+            self.persistent.append(use_async or use_tail)
+            if use_tail:
+                self.sources.append(pass_source_tail(nm, words, n, R, TB, lpos, gen_first and i == 0,
+                                                     measure and i == npass - 1, waves, readout, lanes, lane_off,
+                                                     interleave, wave_sync))
+            elif use_async:
                 self.sources.append(pass_source_async(nm, words, n, R, TB, lpos, measure and i == npass - 1, waves,
                                                       readout, lanes, lane_off, interleave, wave_sync))
             elif self.pipelined:

@@ -10,17 +10,27 @@ server puts in front of ``Circuit.forward``:
   in-flight slot is free.  A request whose batch alone exceeds the cap is
   split into batch chunks that each fit; a single sample that does not fit is
   rejected with an error.
+* Chunks are also bounded to ``max_chunk_frac`` of the cap, so that a large
+  request runs as several chunks that leave room beside them.
 * Order: the smallest waiting chunk first (shortest-footprint first keeps small
   requests from queueing behind large ones), except that chunks which have
   waited longer than ``max_wait_s`` go first, in arrival order.  When such an
   aged chunk does not fit yet, memory is reserved for it: other chunks are only
   admitted while they leave room for it (backfilling), so it is admitted as
   soon as enough in-flight work retires and nothing starves.
-* Admitted work is enqueued at once on one of several HIP streams (default
-  priority) and overlaps on the device with everything else in flight; the
-  host never blocks on a request: completion is detected by polling a HIP
-  event, after which the request's memory is released and more requests are
-  admitted (continuous batching, not a queue drained one batch at a time).
+* Small and large work: a chunk of at most ``small_work`` amplitude updates
+  (milliseconds of device time) is small.  Large chunks may hold at most
+  ``big_share`` of the cap; the rest is a pool that small chunks can always
+  use, an aged reservation notwithstanding, so a burst of large requests does
+  not hold up the many small ones.
+* Admitted work is enqueued at once on a HIP stream (default priority) and
+  overlaps on the device with everything else in flight.  A stream runs its
+  work in order, so small chunks get a stream of their own and each large chunk
+  goes to the other stream with the least outstanding work: a small request
+  never waits behind a large one in a stream.  The host never blocks on a
+  request: completion is detected by polling a HIP event, after which the
+  request's memory is released and more requests are admitted (continuous
+  batching, not a queue drained one batch at a time).
 * Kernels: requests of at least ``compiled_from_qubits`` qubits run the pass
   kernels compiled for their program structure (``precompile`` builds them
   ahead of serving, in worker processes, into the code-object cache); smaller
@@ -58,6 +68,7 @@ class Request:
     n: int = 0
     B: int = 0
     footprint: int = 0
+    work: int = 0                  # amplitude updates per sample
     compiled: bool = False
     chunks: list = dataclasses.field(default_factory=list)
     parts: dict = dataclasses.field(default_factory=dict)
@@ -85,6 +96,8 @@ class _Chunk:
     event: typing.Any = None
     out: typing.Any = None
     keep: typing.Any = None
+    work: int = 0       # amplitude updates: (gates + qubits) x samples x 2^n
+    stream: int = -1    # index of the stream it was enqueued on
 
 
 def state_footprint(n: int, B: int, R: int, TB: int) -> int:
@@ -161,20 +174,25 @@ def precompile(requests, compiled_from_qubits: int, workers: int = 8) -> int:
 
 class ServingEngine:
     def __init__(self, device, cap_bytes: float = 200e9, streams: int = 4, max_inflight: int = 64,
-                 max_wait_s: float = 2.0, compiled_from_qubits: typing.Optional[int] = None):
+                 max_wait_s: float = 2.0, compiled_from_qubits: typing.Optional[int] = None,
+                 max_chunk_frac: float = 0.25, small_work: float = 1e11, big_share: float = 0.9):
         self.device = torch.device(device)
         self.max_wait_s = max_wait_s
         self.cap = int(cap_bytes)
         self.compiled_from = compiled_from_qubits
+        self.max_chunk_frac = max_chunk_frac
+        self.small_work = small_work
+        self.big_cap = int(self.cap * big_share)
         self.streams = [torch.cuda.Stream(self.device) for _ in range(streams)]
+        self.stream_work = [0] * streams
         self.max_inflight = max_inflight
         self.queue: typing.List[_Chunk] = []
         self.inflight: typing.List[_Chunk] = []
         self.resident = 0
+        self.small_resident = 0
         self.peak_resident = 0
         self.done: typing.List[Request] = []
         self.t0 = None
-        self._rr = 0
         L.load()
 
     # ----------------------------------------------------------- admission
@@ -195,11 +213,13 @@ class ServingEngine:
             fused.get_plan(prog, R, TB, fused.INTERPRETED_COST)
         per = state_footprint(req.n, 1, R, TB)
         req.footprint = state_footprint(req.n, req.B, R, TB)
+        req.work = (len(prog.gates) + prog.n) << prog.n  # amplitude updates per sample
         if per > self.cap:
             req.error = f"one sample needs {per} bytes, over the {self.cap}-byte admission cap"
             return req
-        step = req.B if req.footprint <= self.cap else max(1, self.cap // per)
-        while step > 1 and state_footprint(req.n, step, R, TB) > self.cap:
+        limit = max(per, int(self.cap * self.max_chunk_frac))
+        step = req.B if req.footprint <= limit else max(1, limit // per)
+        while step > 1 and state_footprint(req.n, step, R, TB) > limit:
             step -= 1
         req.chunks = [(lo, min(req.B, lo + step)) for lo in range(0, req.B, step)]
         return req
@@ -212,7 +232,7 @@ class ServingEngine:
         req.t_queued = self.now()
         R, TB = kernel_shape(req.n, req.compiled)
         for lo, hi in req.chunks:
-            self.queue.append(_Chunk(req, lo, hi, state_footprint(req.n, hi - lo, R, TB)))
+            self.queue.append(_Chunk(req, lo, hi, state_footprint(req.n, hi - lo, R, TB), work=req.work * (hi - lo)))
         self._admit()
 
     def now(self) -> float:
@@ -227,6 +247,22 @@ class ServingEngine:
         young.sort(key=lambda c: (c.bytes, c.req.t_queued, c.lo))
         return aged + young, len(aged)
 
+    def small(self, ch: _Chunk) -> bool:
+        return ch.work <= self.small_work
+
+    def fits(self, ch: _Chunk, reserve: int) -> bool:
+        """Whether ``ch`` may be admitted now, with ``reserve`` bytes held back for an
+        aged chunk: large chunks stay within their share of the cap; small ones fit
+        beside the reservation or in the small pool (the cap less the large share,
+        which large chunks never take)."""
+        if self.resident + ch.bytes > self.cap:
+            return False
+        if self.small(ch):
+            return (self.resident + ch.bytes + reserve <= self.cap
+                    or self.small_resident + ch.bytes <= self.cap - self.big_cap)
+        big = self.resident - self.small_resident
+        return big + ch.bytes + reserve <= self.big_cap and self.resident + ch.bytes + reserve <= self.cap
+
     def _admit(self):
         """Admit in _order while memory and slots allow; the first aged chunk that
         does not fit reserves its bytes, and later chunks only backfill the rest."""
@@ -234,7 +270,7 @@ class ServingEngine:
             order, naged = self._order()
             reserve, pick = 0, None
             for k, ch in enumerate(order):
-                if self.resident + ch.bytes + reserve <= self.cap or not self.inflight:
+                if self.fits(ch, reserve) or not self.inflight:
                     pick = ch
                     break
                 if reserve == 0 and k < naged:
@@ -264,10 +300,17 @@ class ServingEngine:
         req.t_done = self.now()
         self.done.append(req)
 
+    def pick_stream(self, ch: _Chunk) -> int:
+        """Stream 0 takes the small chunks; a large chunk goes to the stream (1 and
+        up) with the least outstanding work (with one stream, everything is on it)."""
+        if len(self.stream_work) == 1 or self.small(ch):
+            return 0
+        return min(range(1, len(self.stream_work)), key=lambda i: (self.stream_work[i], i))
+
     def _launch(self, ch: _Chunk):
         req = ch.req
-        s = self.streams[self._rr % len(self.streams)]
-        self._rr += 1
+        ch.stream = self.pick_stream(ch)
+        s = self.streams[ch.stream]
         s.wait_stream(torch.cuda.current_stream(self.device))
         with torch.cuda.stream(s), torch.no_grad():
             kw = dict(req.inputs)
@@ -288,9 +331,24 @@ class ServingEngine:
             ch.event = ev
         if req.t_admit < 0:
             req.t_admit = self.now()
+        self._hold(ch)
+
+    def _hold(self, ch: _Chunk):
+        """Account an admitted chunk's memory and stream work."""
         self.resident += ch.bytes
+        if self.small(ch):
+            self.small_resident += ch.bytes
+        if ch.stream >= 0:
+            self.stream_work[ch.stream] += ch.work
         self.peak_resident = max(self.peak_resident, self.resident)
         self.inflight.append(ch)
+
+    def _release(self, ch: _Chunk):
+        self.resident -= ch.bytes
+        if self.small(ch):
+            self.small_resident -= ch.bytes
+        if ch.stream >= 0:
+            self.stream_work[ch.stream] -= ch.work
 
     def poll(self) -> int:
         """Retire finished chunks, release their memory, admit more.  Returns #retired."""
@@ -298,7 +356,7 @@ class ServingEngine:
         for ch in self.inflight:
             if ch.event.query():
                 retired += 1
-                self.resident -= ch.bytes
+                self._release(ch)
                 req = ch.req
                 out, ch.out, ch.keep = ch.out, None, None
                 if req.error is not None:

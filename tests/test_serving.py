@@ -84,14 +84,17 @@ class _DoneEvent:
         return True
 
 
-def _host_engine(cap):
-    """A ServingEngine with its host-side state only (no streams, no device)."""
+def _host_engine(cap, small_work=-1):
+    """A ServingEngine with its host-side state only (no streams, no device); with the
+    default ``small_work`` every chunk counts as large."""
     import time
 
     eng = serving.ServingEngine.__new__(serving.ServingEngine)
     eng.cap, eng.max_inflight, eng.max_wait_s, eng.compiled_from = int(cap), 64, 2.0, None
+    eng.max_chunk_frac, eng.small_work, eng.big_cap = 0.25, small_work, int(cap * 0.9)
+    eng.stream_work = [0] * 4
     eng.queue, eng.inflight, eng.done = [], [], []
-    eng.resident = eng.peak_resident = eng._rr = 0
+    eng.resident = eng.small_resident = eng.peak_resident = 0
     eng.t0 = time.perf_counter()
     return eng
 
@@ -112,8 +115,7 @@ def test_oom_rejection_drops_every_chunk_of_the_request(monkeypatch):
             raise torch.cuda.OutOfMemoryError("fake allocator failure")
         launched.append(ch)
         ch.out, ch.event = torch.full((ch.hi - ch.lo, ch.req.n), 0.5), _DoneEvent()
-        eng.resident += ch.bytes
-        eng.inflight.append(ch)
+        eng._hold(ch)
 
     monkeypatch.setattr(eng, "_launch", fake_launch)
     eng.submit(big)
@@ -150,8 +152,7 @@ def test_admission_order_shortest_first_aging_and_backfill(monkeypatch):
     def fake_launch(ch):
         launched.append(ch.req.rid)
         ch.event, ch.out = _Pending(), torch.zeros(ch.hi - ch.lo, 1)
-        eng.resident += ch.bytes
-        eng.inflight.append(ch)
+        eng._hold(ch)
 
     monkeypatch.setattr(eng, "_launch", fake_launch)
 
@@ -162,15 +163,14 @@ def test_admission_order_shortest_first_aging_and_backfill(monkeypatch):
 
     now = eng.now()
     # 600 B already resident; queue: a young 300, a young 100, an aged 500 (waited 5 s)
-    eng.resident = 600
-    eng.inflight.append(serving._Chunk(serving.Request(99, None, {}), 0, 1, 600, event=_Pending()))
+    eng._hold(serving._Chunk(serving.Request(99, None, {}), 0, 1, 600, event=_Pending()))
     eng.queue = [req(1, 300, now), req(2, 100, now), req(3, 500, now - 5.0)]
     eng._admit()
     # the aged 500 does not fit (600 + 500 > 1000) and reserves 500: only chunks that keep
     # resident + bytes <= 500 may pass, and 600 is already resident, so nothing is admitted
     assert launched == []
     # the resident work retires: the aged chunk goes first, then the smallest young ones that fit
-    eng.inflight, eng.resident = [], 0
+    eng.inflight, eng.resident, eng.small_resident = [], 0, 0
     eng._admit()
     assert launched == [3, 2, 1][: len(launched)] and launched[0] == 3 and launched[1] == 2
     assert eng.resident <= 1000
@@ -179,6 +179,57 @@ def test_admission_order_shortest_first_aging_and_backfill(monkeypatch):
 class _Pending:
     def query(self):
         return False
+
+
+def test_small_pool_and_stream_choice(monkeypatch):
+    """Small chunks (few amplitude updates) may use the pool that large chunks
+    never take even while an aged large chunk holds a reservation; large chunks
+    stay within their share; each large chunk goes to the stream with the least
+    outstanding work, small ones to stream 0.  Chunks of a large request are
+    bounded to a quarter of the cap."""
+    eng = _host_engine(1000, small_work=50)
+    launched = []
+
+    def fake_launch(ch):
+        ch.stream = eng.pick_stream(ch)
+        launched.append((ch.req.rid, ch.stream))
+        ch.event, ch.out = _Pending(), torch.zeros(ch.hi - ch.lo, 1)
+        eng._hold(ch)
+
+    monkeypatch.setattr(eng, "_launch", fake_launch)
+
+    def chunk(rid, nbytes, work, queued):
+        r = serving.Request(rid, None, {})
+        r.n, r.B, r.t_queued, r.chunks = 1, 1, queued, [(0, 1)]
+        return serving._Chunk(r, 0, 1, nbytes, work=work)
+
+    now = eng.now()
+    # large work of 850 B in flight (big share 900): an aged large 200 reserves; a young large
+    # 40 would cross the share, a young small 60 fits the small pool (1000 - 900) regardless
+    for b, w in ((400, 1000), (450, 500)):
+        fake_launch(chunk(90 + b, b, w, now))
+    assert [s for _r, s in launched] == [1, 2]
+    launched.clear()
+    eng.queue = [chunk(1, 200, 10 ** 6, now - 5.0), chunk(2, 40, 100, now), chunk(3, 60, 10, now)]
+    eng._admit()
+    assert launched == [(3, 0)]
+    assert eng.small_resident == 60 and eng.resident == 910
+    # the first large chunk retires: the aged one goes to the emptied stream 1 (streams 1 and 3
+    # hold no work; ties go to the lower index), the young large one then to stream 3
+    first = eng.inflight[0]
+    eng.inflight.remove(first)
+    eng._release(first)
+    eng._admit()
+    assert launched[1:] == [(1, 1), (2, 3)]
+    assert eng.resident - eng.small_resident <= eng.big_cap
+    # chunk bound: a request of 37 samples of 12 qubits under a cap of 10 samples' bytes
+    eng.cap = 10 * (2 ** 12) * 8 + 50_000
+    r, _ = _req(0, 12, 37, 2, 1)
+    eng.prepare(r)
+    sizes = [hi - lo for lo, hi in r.chunks]
+    assert all(serving.state_footprint(12, s, 5, 6) <= max(eng.cap // 4, serving.state_footprint(12, 1, 5, 6))
+               for s in sizes)
+    assert r.chunks[-1][1] == 37 and len(r.chunks) >= 4
 
 
 def test_precompile_workers_fill_the_code_cache(tmp_path, monkeypatch):

@@ -249,3 +249,44 @@ def test_async_pass_sources_compile(tmp_path):
         else:
             i += 1
     assert ops and waits and blocks >= 2
+
+
+def test_tail_direct_stores_match_the_tile_layout():
+    """In the tile-streaming pass kernels the last sub-pass stores register j of
+    thread tq to gbase | sum of 1 << GP[r] over j's bits, where the tile write-back
+    would put it at tile element gl ^ (xor of 1 << P[r]) and from there at
+    scatter(element): for every pass of a 20-qubit program the two addresses agree,
+    and over all threads and registers each tile amplitude is stored exactly once."""
+    from qandle_amd import workloads
+    from qandle_amd._engine import fused
+
+    n, R, TB = 20, 5, 8
+    K = R + TB
+    w = workloads.rxryrz_ladder(n=n, B=2, depth=4, seed=3)
+    circ = qandle.Circuit(w.layers, num_qubits=n)
+    prog = fused.lower_circuit(circ.circuit)
+    lead = fused.leading_rotations(prog)
+    p = lead[0] if lead else prog
+    pl = fused.get_plan(p, R, TB, None, True)
+    ws = fused.pass_words(p, pl, 2, "cpu")
+    for wd, _ns, lpos in ws.passes:
+        lpos = [int(x) for x in lpos]
+        scat = lambda l: sum(1 << lpos[k] for k in range(K) if (l >> k) & 1)  # noqa: E731
+        sub = PL.decode_pass(wd.numpy(), R, TB)[-1]
+        seen = []
+        for tq in range(1 << TB):
+            gl = sum(((tq >> i) & 1) << sub.G[i] for i in range(TB))
+            gbase = sum(((tq >> i) & 1) << sub.GG[i] for i in range(TB))
+            for j in range(1 << R):
+                el = gl
+                off = gbase
+                for r in range(R):
+                    if (j >> r) & 1:
+                        el ^= 1 << sub.P[r]
+                        off |= 1 << sub.GP[r]
+                assert off == scat(el)
+                seen.append(off)
+        assert sorted(seen) == sorted(scat(l) for l in range(1 << K))
+        # lane bits: the lowest tile bits that are neither registers nor wave bits, ascending
+        taken = set(sub.P[:R]) | set(sub.G[6:TB])
+        assert [sub.G[i] for i in range(6)] == [b for b in range(K) if b not in taken][:6]
