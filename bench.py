@@ -252,6 +252,59 @@ def pass_bytes(B, n, npass, stores_last):
     return [amps * 8] + [amps * 16] * (npass - 2) + [amps * (16 if stores_last else 8)]
 
 
+MFMA_PROBE = r"""
+typedef float qs_f4 __attribute__((ext_vector_type(4)));
+extern "C" __global__ void __launch_bounds__(256) qs_mfma_probe(float *out, int iters, float a0) {
+  qs_f4 c[8];
+  for (int k = 0; k < 8; ++k) c[k] = qs_f4{0.f, 0.f, 0.f, 0.f};
+  const float a = a0 + threadIdx.x * 1e-7f, b = a0 * 0.5f;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c[k] = __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c[k], 0, 0, 0);
+  }
+  float s = 0.f;
+  for (int k = 0; k < 8; ++k) s += c[k].x + c[k].y + c[k].z + c[k].w;
+  out[blockIdx.x * 256 + threadIdx.x] = s;
+}
+"""
+
+
+def mfma_sustained(dev, stream):
+    """The fp32 matrix-core rate this GPU sustains, measured: 8 independent chains
+    of v_mfma_f32_4x4x1 (the pass kernels' instruction) per wave, with 2 and with
+    8 waves per SIMD (2 is the pass kernels' occupancy).  The published 157.3
+    TFLOP/s assumes the peak engine clock; under matrix-core load the clock runs
+    lower, and this is the rate the pass kernels can approach."""
+    import ctypes
+    import struct as st
+
+    import torch
+
+    from qandle_amd._engine import jit, lib as L
+
+    fn = jit.kernels_for([MFMA_PROBE], ["qs_mfma_probe"])[0]
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    out = {}
+    for waves in (2, 8):
+        blocks = waves * cus                      # 4 waves per 256-thread block, 4 SIMDs per CU
+        iters = max(64, int(15000 * 2 / waves))
+        buf_t = torch.empty(blocks * 256, dtype=torch.float32, device=dev)
+        best = None
+        for rep in range(4):
+            args = st.pack("<Qif", buf_t.data_ptr(), iters if rep else 16, 1.0)
+            buf = ctypes.create_string_buffer(args, len(args))
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            L.check(L.load().qs_jit_launch(fn, blocks, 256, buf, len(args), stream.cuda_stream), "qs_jit_launch")
+            e1.record(stream)
+            torch.cuda.synchronize(dev)
+            if rep:
+                t = e0.elapsed_time(e1) / 1e3
+                best = t if best is None else min(best, t)
+        out[f"{waves}_waves_per_simd"] = blocks * 4 * iters * 8 * 512 / best / 1e12
+    return out
+
+
 def hbm_roofline(events, kernel):
     """HBM roofline dict from (start, end, bytes) launch records."""
     durs = [a.elapsed_time(b) / 1e3 for a, b, _ in events]
@@ -388,9 +441,12 @@ def main():
     units = sum(u for u, _ in struct)
     flops_per_launch = units / npass * B * 2 ** n * 16
     tflops = flops_per_launch / (roof["avg_launch_us"] / 1e6) / 1e12
+    sustained = mfma_sustained(dev, stream)
     roof["mfma"] = {"achieved": tflops, "peak": MFMA_F32_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": tflops / MFMA_F32_PEAK_TFLOPS, "alg_flops_per_launch": flops_per_launch,
-                    "fused_unitaries_per_step": units}
+                    "fused_unitaries_per_step": units,
+                    "sustained_measured": sustained,
+                    "frac_of_sustained": tflops / sustained["2_waves_per_simd"]}
     roof["per_pass"] = per_pass
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_fused_pass_n20_b64.json")
@@ -562,6 +618,7 @@ def bench_uneven(dev, time_steps):
 
 
 SERVE_COMPILED_FROM = 18  # requests of this many qubits and more run compiled kernels (99.8% of the mix's work)
+SERVE_FIXED_RATE_HZ = 7.5  # a fixed offered load of the request mix, for comparing serving runs
 
 
 SERVE_MIX = dict(num=160, seed=4)
@@ -612,6 +669,11 @@ def bench_serving(dev, warm, num=SERVE_MIX["num"], seed=SERVE_MIX["seed"]):
     done = eng.serve(reqs, timeout_s=300)
     wall = time.perf_counter() - t0
     st = serving.latency_stats(done)
+    # the same requests at a fixed offered load, so that runs compare at the same load
+    reqs, eng = fresh()
+    for r, a in zip(reqs, arrivals):
+        r.arrival = a / SERVE_FIXED_RATE_HZ
+    fixed = serving.latency_stats(eng.serve(reqs, timeout_s=300))
     return {
         "serve_requests": st["requests"],
         "serve_rejected": st["rejected"],
@@ -628,6 +690,10 @@ def bench_serving(dev, warm, num=SERVE_MIX["num"], seed=SERVE_MIX["seed"]):
         "serve_updates_per_s": work / wall,
         "serve_peak_resident_gb": eng.peak_resident / 1e9,
         "serve_max_qubits": int(max(r.n for r in reqs)),
+        "serve_fixed_rate_hz": SERVE_FIXED_RATE_HZ,
+        "serve_fixed_p50_ms": fixed["p50_ms"],
+        "serve_fixed_p99_ms": fixed["p99_ms"],
+        "serve_fixed_service_p50_ms": fixed["service_p50_ms"],
     }
 
 
