@@ -139,14 +139,19 @@ def profile_workload(mode, reps=4):
     w = headline_workload() if mode == "fused" else adjoint_workload()
     circ = qandle.Circuit(w.layers, num_qubits=w.n).to(dev)
     x = w.inputs["x"].to(dev)
+    xg = x.clone().requires_grad_(True)
+    lw = w.loss_w.to(dev)
+    params = list(circ.parameters())
 
     def step():
         if mode == "fused":
             with torch.no_grad():
                 circ(x=x)
-        else:
-            xg = x.clone().requires_grad_(True)
-            (circ(x=xg) * w.loss_w.to(dev)).sum().backward()
+        else:  # the adjoint leg's step: gradients cleared as an optimizer does, then forward + backward
+            for p in params:
+                p.grad = None
+            xg.grad = None
+            (circ(x=xg) * lw).sum().backward()
 
     settle(step, sync)
     for _ in range(reps):
@@ -550,15 +555,21 @@ def bench_adjoint(dev, time_steps, sync):
     c3 = qandle.Circuit(w.layers, num_qubits=w.n).to(dev)
     x3 = w.inputs["x"].to(dev).requires_grad_(True)
     lw = w.loss_w.to(dev)
+    # gradients are cleared as an optimizer's zero_grad(set_to_none=True) does, over the
+    # parameter list taken once (Module.zero_grad walks the 960 gate modules every call)
+    params = list(c3.parameters())
+
+    def clear():
+        for p in params:
+            p.grad = None
+        x3.grad = None
 
     def adj_step():
-        c3.zero_grad(set_to_none=True)
-        x3.grad = None
+        clear()
         (c3(x=x3) * lw).sum().backward()
 
     def plain_adj_step():
-        c3.zero_grad(set_to_none=True)
-        x3.grad = None
+        clear()
         (c3.circuit.forward_layers(x=x3) * lw).sum().backward()
 
     settle(adj_step, sync)

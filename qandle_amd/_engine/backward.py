@@ -71,8 +71,10 @@ class FusedCircuitFn(torch.autograd.Function):
         ckpts = {}
 
         def keep(i, psi):
-            if i in marks:
-                ckpts[i] = psi.clone()
+            if i in marks:  # the pass's output stays as the checkpoint; later passes write a new buffer
+                ckpts[i] = psi
+                return torch.empty_like(psi)
+            return None
 
         psi, probs = fused.execute(prog, inp, cs, efac, R, TB, cm=cm, want_state=True, checkpoints=keep,
                                    specialise=jit_on, fold=fold)
@@ -96,7 +98,10 @@ class FusedCircuitFn(torch.autograd.Function):
         n = prog.n
         lib = L.load()
         st = L.stream_of(psi_final)
-        psi = psi_final.clone()
+        # the compiled steps read psi from psi_src and write psi: the first reads the saved final
+        # state and each reset reads its checkpoint, with no copies; the interpreting steps work in place
+        psi = torch.empty_like(psi_final) if ctx.jit else psi_final.clone()
+        psi_src = psi_final
         lam = torch.empty_like(psi)
         seed = None
         if prog.measure:
@@ -133,7 +138,8 @@ class FusedCircuitFn(torch.autograd.Function):
                 ev[0].record(torch.cuda.current_stream(psi.device))
             if kern is not None:
                 kern.launch(k, L.ptr(psi), L.ptr(lam), L.ptr(utab), L.ptr(qtab), L.ptr(cs), L.ptr(gpart), seed_ptr,
-                            B, NS, ntiles, st, None if ltab is None else L.ptr(ltab))
+                            B, NS, ntiles, st, None if ltab is None else L.ptr(ltab), L.ptr(psi_src))
+                psi_src = psi
             else:
                 L.check(
                     lib.qs_adjoint_pass(
@@ -149,7 +155,10 @@ class FusedCircuitFn(torch.autograd.Function):
                 events.append((ev[0], ev[1], B * (2 ** n) * (24 if seed_ptr is not None else 32)))
             i = npass - 1 - k  # forward pass this step undid
             if k < nmain and (i - 1) in ckpts:
-                psi.copy_(ckpts[i - 1])
+                if kern is not None:
+                    psi_src = ckpts[i - 1]
+                else:
+                    psi.copy_(ckpts[i - 1])
         per = torch.empty(max(total_rows, 1), B, dtype=torch.float32, device=psi.device)
         tot = torch.empty(max(total_rows, 1), dtype=torch.float32, device=psi.device)
         L.check(lib.qs_adjoint_finish(L.ptr(per), L.ptr(tot), L.ptr(gpart), total_rows, B, n, R, TB, st),

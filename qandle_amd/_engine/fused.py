@@ -798,8 +798,16 @@ def forward_kernels(prog: Program, B: int, R: int, TB: int, cm: PL.CostModel = N
 
 def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
             cm: PL.CostModel = None, want_state: bool = True, checkpoints=None, events=None,
-            specialise: bool = None, fold: bool = None):
+            specialise: bool = None, fold: bool = None, out: torch.Tensor = None):
     """Runs every pass.  Returns (state or None, probs or None).
+
+    ``checkpoints(i, state)``: called after pass i with the state it wrote; when
+    it returns a tensor (same shape), the passes continue in that one and the
+    caller keeps ``state`` as it is (a checkpoint without a copy).
+    ``out``: the (B, 2^n) complex64 device tensor the passes run in (the
+    returned state); None allocates it.  A caller that launches on a side
+    stream can take it from the pool of the stream it allocates on (and
+    record it on the side stream), so one pool serves every stream.
 
     ``events``: optional list; a (start, end) pair of HIP events recorded on the
     launch stream around every pass kernel is appended to it.  ``specialise``:
@@ -833,7 +841,13 @@ def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
     lib = L.load()
     st = L.stream_of(cs)
     utab, _ = compose_tables(ws, cs, generators=False)
-    psi = torch.empty(B, 2 ** n, dtype=torch.complex64, device=inp.device)
+    if out is not None:
+        if (out.dtype != torch.complex64 or tuple(out.shape) != (B, 2 ** n) or not out.is_contiguous()
+                or out.device != inp.device):
+            raise ValueError(f"out must be a contiguous ({B}, {2 ** n}) complex64 tensor on {inp.device}")
+        psi = out
+    else:
+        psi = torch.empty(B, 2 ** n, dtype=torch.complex64, device=inp.device)
     mpart = probs = None
     if prog.measure:
         wb = (lib.qs_readout_work_bytes if fold else lib.qs_fused_measure_work_bytes)(B, n, R, TB)
@@ -877,7 +891,10 @@ def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,
             events.append(ev)
         src = psi
         if checkpoints is not None:
-            checkpoints(i, psi)
+            nxt = checkpoints(i, psi)
+            if nxt is not None:
+                # the caller keeps this pass's output; the next pass reads it and writes nxt
+                psi = nxt
     if prog.measure and fold:
         probs = torch.empty(B, n, dtype=torch.float32, device=inp.device)
         L.check(lib.qs_readout_finish(L.ptr(probs), L.ptr(mpart), B, n, R, TB, st), "qs_readout_finish")

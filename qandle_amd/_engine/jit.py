@@ -198,6 +198,7 @@ __device__ __forceinline__ float half_sum(float s) {
 struct JAdj {
   float *psi; float *lam; const int *utab; const int *qtab; const int *cs; float *gpart; const float *seed_w;
   const float *ltab;
+  const float *psi_src;  // where the step reads psi (psi itself, or a checkpoint): it writes psi
   i64 B; i64 NS; i64 ntiles;
 };
 // vector-ALU form with the unitary read through the scalar path (uniform tables)
@@ -222,7 +223,7 @@ __device__ __forceinline__ void valu_pair(float2 &x, float2 &y, const int *u) {
 """
 
 ARGS_FMT = "<QQQQQQQqqii"  # struct JArgs
-ADJ_FMT = "<QQQQQQQQqqq"   # struct JAdj
+ADJ_FMT = "<QQQQQQQQQqqq"  # struct JAdj
 OPTIONS = ["--offload-arch=gfx950", "-O3", "-std=c++17",
            "-mllvm", "-simplifycfg-sink-common=false",
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -1077,7 +1078,7 @@ def adjoint_source(name: str, words, n: int, R: int, TB: int, lpos, seed: bool,
     e("  auto fetch = [&](i64 base) {")
     for it in range(NR // 2):
         e(f"    {{ const i64 gi = base | g_tid | {goff[it]}LL;")
-        ld_a = "reinterpret_cast<const float4 *>(A.psi)[gi >> 1]"
+        ld_a = "reinterpret_cast<const float4 *>(A.psi_src)[gi >> 1]"
         ld_b = "reinterpret_cast<const float4 *>(A.lam)[gi >> 1]"
         if multi:
             e(f"      pa[{it}] = gi < total ? {ld_a} : make_float4(0.f, 0.f, 0.f, 0.f);")
@@ -1557,7 +1558,8 @@ class AdjointKernels:
         return cache[key]
 
     def launch(self, i: int, psi, lam, utab, qtab, cs, gpart, seed_w, B: int, NS: int, ntiles: int, stream,
-               ltab=None):
+               ltab=None, psi_src=None):
+        """``psi_src``: where the step reads psi (None: psi itself); it writes psi."""
         lib = L.load()
         if self.full_grid:
             grid = max(1, min(ntiles, 0x7fffffff))
@@ -1566,7 +1568,7 @@ class AdjointKernels:
                 g = lib.qs_jit_resident_blocks(self.funcs[i], self.block)
                 self.grid_cap[i] = int(g) if g and g > 0 else 2048
             grid = max(1, min(ntiles, self.grid_cap[i]))
-        args = struct.pack(ADJ_FMT, psi, lam, utab or 0, qtab or 0, cs or 0, gpart, seed_w or 0, ltab or 0, B, NS,
-                           ntiles)
+        args = struct.pack(ADJ_FMT, psi, lam, utab or 0, qtab or 0, cs or 0, gpart, seed_w or 0, ltab or 0,
+                           psi_src or psi, B, NS, ntiles)
         buf = ctypes.create_string_buffer(args, len(args))
         L.check(lib.qs_jit_launch(self.funcs[i], grid, self.block, buf, len(args), stream), "qs_jit_launch")

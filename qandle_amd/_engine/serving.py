@@ -312,6 +312,12 @@ class ServingEngine:
         ch.stream = self.pick_stream(ch)
         s = self.streams[ch.stream]
         s.wait_stream(torch.cuda.current_stream(self.device))
+        # the state comes from the allocating stream's pool and is recorded on the launch stream:
+        # one pool then serves the chunks of every stream (a block freed by a chunk on one stream is
+        # reused by the next large chunk on any other once its work is done), instead of each stream
+        # caching its own blocks and new ones being mapped for every large chunk
+        psi = torch.empty(ch.hi - ch.lo, 2 ** req.n, dtype=torch.complex64, device=self.device)
+        psi.record_stream(s)
         with torch.cuda.stream(s), torch.no_grad():
             kw = dict(req.inputs)
             prog = fused.lower_circuit(req.circuit.circuit)
@@ -323,7 +329,8 @@ class ServingEngine:
             cs = fused.build_coeffs(prog, inp)
             efac = fused.product_factors(prog, inp)
             cm = fused.DEFAULT_COST if req.compiled else fused.INTERPRETED_COST
-            _, probs = fused.execute(prog, inp, cs, efac, R, TB, cm=cm, want_state=False, specialise=req.compiled)
+            _, probs = fused.execute(prog, inp, cs, efac, R, TB, cm=cm, want_state=False, specialise=req.compiled,
+                                     out=psi)
             ch.out = probs
             ch.keep = (cs, efac)
             ev = torch.cuda.Event()
