@@ -573,8 +573,10 @@ def bench_adjoint(dev, time_steps, sync):
         (c3.circuit.forward_layers(x=x3) * lw).sum().backward()
 
     settle(adj_step, sync)
-    ta, _ = time_steps(adj_step, 6, 2)
-    log(f"adjoint: {ta / 6 * 1e3:.2f} ms per forward + backward; adjoint kernel launches, then plain autograd")
+    blocks = sorted(time_steps(adj_step, 10, 2)[0] / 10 for _ in range(3))
+    ta = blocks[1] * 6  # the median block, per 6 steps (below)
+    log(f"adjoint: {blocks[1] * 1e3:.2f} ms per forward + backward (median of 3 blocks of 10 steps); "
+        "adjoint kernel launches, then plain autograd")
     events = []
     backward.launch_events = events
     try:
@@ -585,7 +587,8 @@ def bench_adjoint(dev, time_steps, sync):
         backward.launch_events = None
     roof = hbm_roofline(events, "qs_adj_<i> (adjoint sweep steps specialised per step program, hipRTC)")
     tp, _ = time_steps(plain_adj_step, 1, 1)
-    out = {"adjoint_fwd_bwd_ms": ta / 6 * 1e3, "plain_autograd_fwd_bwd_ms": tp * 1e3,
+    out = {"adjoint_fwd_bwd_ms": ta / 6 * 1e3, "adjoint_fwd_bwd_blocks_ms": [b * 1e3 for b in blocks],
+           "plain_autograd_fwd_bwd_ms": tp * 1e3,
            "adjoint_speedup_vs_plain": tp / (ta / 6), "adjoint_roofline": roof}
     del c3, x3
     return out

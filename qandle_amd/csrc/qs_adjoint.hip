@@ -252,19 +252,42 @@ struct AdjFinishArgs {
   int rows;
 };
 
+// The slots of sample b are contiguous (sps of them, a power of two): each wave reads 64 consecutive
+// slots per load and sums them in segments of sps lanes with shuffles (in double), so the loads are
+// coalesced; with more than 64 slots per sample a wave takes one sample at a time.  The loop trip
+// counts are uniform per wave, so every lane takes part in every shuffle.
 __device__ __forceinline__ void adjoint_finish_row(const AdjFinishArgs &F, int64_t row, double *red) {
   const float *g = F.gpart + row * F.NS;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   double all = 0.0;
-  for (int64_t b = threadIdx.x; b < F.B; b += 256) {
-    double acc = 0.0;
-    for (int64_t k = 0; k < F.sps; ++k) acc += g[b * F.sps + k];
-    F.out[row * F.B + b] = (float)(0.5 * acc);
-    all += 0.5 * acc;
+  if (F.sps <= 64) {
+    const int64_t total = F.B * F.sps;
+    const int seg = (int)F.sps;
+    for (int64_t c = (int64_t)wave * 64; c < total; c += 256) {
+      const int64_t i = c + lane;
+      double acc = i < total ? (double)g[i] : 0.0;
+      for (int off = 1; off < seg; off <<= 1) acc += __shfl_xor(acc, off, 64);
+      if ((lane & (seg - 1)) == 0 && i < total) {
+        F.out[row * F.B + i / seg] = (float)(0.5 * acc);
+        all += 0.5 * acc;
+      }
+    }
+  } else {
+    for (int64_t b = wave; b < F.B; b += 4) {
+      const float *gb = g + b * F.sps;
+      double acc = 0.0;
+      for (int64_t k = lane; k < F.sps; k += 64) acc += (double)gb[k];
+      for (int off = 1; off < 64; off <<= 1) acc += __shfl_xor(acc, off, 64);
+      if (lane == 0) {
+        F.out[row * F.B + b] = (float)(0.5 * acc);
+        all += 0.5 * acc;
+      }
+    }
   }
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) all += __shfl_xor(all, off, 64);
   __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = all;
+  if (lane == 0) red[wave] = all;
   __syncthreads();
   if (threadIdx.x == 0) F.tot[row] = (float)(red[0] + red[1] + red[2] + red[3]);
 }
