@@ -752,15 +752,15 @@ def cpu_baseline(w):
     """qandle's complex64 torch semantics restated (oracle/qandle_c64.py, structured
     mode: the dense reference needs 2 x 8 TiB of cached matrices per gate at 20
     qubits) on a bounded sample of the headline workload: the full 810-gate circuit
-    on 4 of its 64 samples, median of 5 runs after a warm-up, on every CPU of this
-    process's share."""
+    on 12 of its 64 samples (10-20 s of CPU work over the runs), median of 5 runs
+    after a warm-up, on every CPU of this process's share."""
     import torch
 
     from oracle import qandle_c64
 
     threads = cpu_share()
     torch.set_num_threads(threads)
-    Bs = 4
+    Bs = 12
     ops = w.torch_ops()
     ops[0] = (ops[0][0], ops[0][1], ops[0][2], ops[0][3][:Bs])
     med, runs = median_time(lambda: qandle_c64.structured_run(w.n, ops, Bs))
