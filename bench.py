@@ -128,7 +128,7 @@ def profile_workload(mode, reps=4):
         from qandle_amd._engine.uneven import UnevenBatch
 
         ws, lw = uneven_workloads()
-        ub = UnevenBatch.from_workloads(ws, dev, R=4, TB=8, cost=fused.PL.CostModel(budget=160.0, min_low=4))
+        ub = UnevenBatch.from_workloads(ws, dev, R=4, TB=8, cost=fused.UNEVEN_COST)
         ub.set_loss_weights(lw)
         settle(ub.run_forward, sync)
         for _ in range(reps):
@@ -607,7 +607,7 @@ def bench_uneven(dev, time_steps):
         ub.set_loss_weights(lw)
         return ub
 
-    ub, table = autotune.tune_uneven(make, cands=((4, 7, 160.0), (4, 8, 160.0)), reps=2)
+    ub, table = autotune.tune_uneven(make, cands=((4, 8, 1024.0), (4, 8, 2048.0), (4, 7, 1024.0)), reps=2)
     ub.capture(backward=False)
     tf, _ = time_steps(lambda: ub.replay(backward=False), 5, 1)
     tfb, _ = time_steps(lambda: ub.replay(backward=True), 5, 1)

@@ -275,6 +275,10 @@ DEFAULT_COST = PL.CostModel(budget=1024.0, min_low=4, lookahead=True)
 # and measured no faster (20.5 vs 17-21 ms at 16 qubits x 256 x 20 layers), so the greedy schedule
 GRAD_COST = PL.CostModel(budget=1024.0, min_low=4)
 INTERPRETED_COST = PL.CostModel(budget=256.0, min_low=4)
+# uneven batches (interpreting item kernels, one launch per level): passes packed to the structural
+# limit, so fewer levels: 4.41 vs 6.25 ms per forward of the 2048-circuit batch (budget 160), 24.1 vs
+# 30.8 ms forward + backward; 5 levels instead of 13
+UNEVEN_COST = PL.CostModel(budget=1024.0, min_low=4)
 
 
 def choose_config(prog: Program, inp, grad: bool = False):

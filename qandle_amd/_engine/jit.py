@@ -248,7 +248,8 @@ def lane_entries(words, R: int, TB: int, multi: bool):
 def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, measure: bool,
                 prefetch: bool = False, waves: int = None, readout=None, valu_pairs: int = 0,
                 lanes=False, split_pairs: bool = False, lane_off: int = 0, interleave: bool = False,
-                wave_sync: bool = False, stagger: int = 0, stagger_cu: int = 256) -> str:
+                wave_sync: bool = False, stagger: int = 0, stagger_cu: int = 256, direct_store: bool = False,
+                direct_load: bool = False) -> str:
     """HIP source of one specialised forward pass kernel (extern "C" ``name``).
 
     ``prefetch`` keeps the next tile's HBM loads in flight in registers while
@@ -277,10 +278,17 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
     ``wave_sync``: between two sub-passes that give the thread-index bits
     above the lane bits (the wave index) the same tile bits, each wave works
     on its own part of the tile and a wave-level fence replaces the block
-    barrier (wave_local)."""
+    barrier (wave_local).  ``direct_store``: the last sub-pass stores its
+    registers straight to the state (as pass_source_tail does) instead of
+    through the tile; ``direct_load``: the first sub-pass loads its registers
+    straight from the state (no staging through the tile); both for tiles
+    within one sample, and direct_load for one tile per workgroup (no
+    prefetch)."""
     K = R + TB
     T, NR = 1 << TB, 1 << R
     multi = n < K
+    direct_store = direct_store and not multi
+    direct_load = direct_load and not multi and not gen and not prefetch
     subs = PL.decode_pass(words, R, TB)
     lpos = [int(x) for x in lpos]
     npos = [] if multi else [p for p in range(n) if p not in lpos]
@@ -336,15 +344,15 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
         e("  if (t < A.ntiles) fetch(tile_base(t));")
     e("  for (; t < A.ntiles; t += gridDim.x) {")
     e("    const i64 base = tile_base(t);")
-    if not gen and not pf:
+    if not gen and not pf and not direct_load:
         e("    fetch(base);")
     e("    const int ts = pinv(tid);")
     if lane_k:
         nl = 4 * len(lane_k)
         e(f"    for (int c = ts; c < {nl}; c += {T}) ctab[c] = reinterpret_cast<const float4 *>(A.ltab)[{4 * lane_off} + c];")
-        if gen:
+        if gen or direct_load:
             e("    __syncthreads();")
-    if not gen:
+    if not gen and not direct_load:
         for it in range(NR // 2):
             if split_pairs:
                 e(f"    {{ const int sl = swz(2 * ({it * T} + ts)); const float4 val = pre[{it}];")
@@ -361,7 +369,13 @@ def pass_source(name: str, words, n: int, R: int, TB: int, lpos, gen: bool, meas
     for s, sub in enumerate(subs):
         _sub_pass(e, sub, s, len(subs), n, R, TB, multi, gen and s == 0, measure and s == len(subs) - 1, readout,
                   valu_pairs, lanes=lane_k if lanes == "lds" else lanes, interleave=interleave,
-                  wave_sync=wave_sync and s + 1 < len(subs) and wave_local(sub, subs[s + 1], TB))
+                  wave_sync=wave_sync and s + 1 < len(subs) and wave_local(sub, subs[s + 1], TB),
+                  tail=[] if direct_store and s == len(subs) - 1 else None, direct_load=direct_load and s == 0)
+    if direct_store:
+        e("    __syncthreads();")
+        e("  }")
+        e("}")
+        return "\n".join(o)
     e("    if (A.store) {")
     e("      const int tz = pinv(tid);")
     for it in range(NR // 2):
@@ -708,14 +722,16 @@ def wave_local(a, b, TB: int) -> bool:
 
 
 def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu_pairs=0, pipe=None, lanes=False,
-              interleave=False, wave_sync=False, swz_mask=31, tail=None):
+              interleave=False, wave_sync=False, swz_mask=31, tail=None, direct_load=False):
     """One sub-pass.  ``pipe`` (pipelined kernels): a dict; the tile is
     ``buf[boff + i]``, no closing barrier is emitted, and a measurement's
     cross-wave finish is left in ``pipe["finish"]`` for after the caller's
-    barrier.  ``tail`` (the last sub-pass of pass_source_tail): a list of lines
-    emitted after a block barrier that follows the register loads (the tile
-    buffer is then free), and the results go straight from registers to the
-    state instead of back into the tile."""
+    barrier.  ``tail`` (the last sub-pass of a kernel that stores from
+    registers): the results go straight from registers to the state instead of
+    back into the tile; a non-empty list of lines is emitted after a block
+    barrier that follows the register loads (the tile buffer is then free).
+    ``direct_load`` (the first sub-pass of a kernel that loads into registers):
+    the amplitudes come straight from the state (``A.src``) instead of the tile."""
     if pipe is not None:
         import re
 
@@ -784,13 +800,15 @@ def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu
         for j in range(NR):
             e(f"        v[{j}] = make_float2(0.f, 0.f);")
         e("      }")
+    elif direct_load:
+        _direct_load(e, sub, R)
     else:
         for j in range(NR):
             if pipe is None:
                 e(f"      v[{j}] = lds_at(tile, lb8 ^ {8 * swj[j]});")
             else:
                 e(f"      v[{j}] = tile[lb ^ {swj[j]}];")
-    if tail is not None:
+    if tail:
         e("      __syncthreads();  // every thread holds its amplitudes: the tile buffer is free")
         for line in tail:
             e(line)
@@ -897,6 +915,34 @@ def _sub_pass(e, sub, s, nsub, n, R, TB, multi, gen, measure, readout=None, valu
     e("    }")
 
 
+def _register_offsets(sub, R):
+    """Amplitude offset (from the thread's gbase) of each register of a sub-pass, and
+    the register bit at amplitude bit 0 (None if none is)."""
+    roff = [sum(1 << sub.GP[r] for r in range(R) if (j >> r) & 1) for j in range(1 << R)]
+    return roff, next((r for r in range(R) if sub.GP[r] == 0), None)
+
+
+def _direct_load(e, sub, R):
+    """Load a sub-pass's registers straight from the state: register j of a thread
+    is amplitude gbase | sum of 1 << GP[r] over its set bits r (the lane bits are the
+    lowest non-register tile bits, so a wave's loads form contiguous runs; with a
+    register bit at amplitude bit 0, two registers come from one 16-byte load)."""
+    NR = 1 << R
+    roff, r0 = _register_offsets(sub, R)
+    e("      {")
+    e("        const float4 *i4 = reinterpret_cast<const float4 *>(A.src);")
+    e("        const float2 *i2 = reinterpret_cast<const float2 *>(A.src);")
+    for j in range(NR):
+        if r0 is not None:
+            if (j >> r0) & 1:
+                continue
+            e(f"        {{ const float4 q = i4[(gbase | {roff[j]}LL) >> 1]; v[{j}] = make_float2(q.x, q.y); "
+              f"v[{j | (1 << r0)}] = make_float2(q.z, q.w); }}")
+        else:
+            e(f"        v[{j}] = i2[gbase | {roff[j]}LL];")
+    e("      }")
+
+
 def _direct_store(e, sub, phys, R):
     """Store a sub-pass's registers straight to the state: register j of a thread
     is amplitude gbase | sum of 1 << GP[r] over its set bits r.  The lane bits hold
@@ -904,8 +950,7 @@ def _direct_store(e, sub, phys, R):
     a wave's stores form contiguous runs; when a register bit sits at amplitude
     bit 0, two registers make one 16-byte pair store."""
     NR = 1 << R
-    roff = [sum(1 << sub.GP[r] for r in range(R) if (j >> r) & 1) for j in range(NR)]
-    r0 = next((r for r in range(R) if sub.GP[r] == 0), None)
+    roff, r0 = _register_offsets(sub, R)
     e("      if (A.store) {")
     e("        float4 *o4 = reinterpret_cast<float4 *>(A.state);")
     e("        float2 *o2 = reinterpret_cast<float2 *>(A.state);")
@@ -1426,6 +1471,15 @@ def lane_mode(lanes, pipelined: bool):
     return "lds" if lanes == "lds" else True
 
 
+# A pass that applies at least this many fused unitaries loads its first sub-pass's
+# registers straight from the state instead of staging the tile through LDS: such a pass
+# is bound by its arithmetic, and the staging's LDS traffic and barrier sit on that path
+# (20 qubits x 64: 437 vs 463 us for a 38-unitary pass, 707 vs 755 us for a 68-unitary one,
+# results bit for bit the same), while streaming-bound passes lose by it (257 vs 197 us for
+# 12 unitaries: its 8-byte loads keep fewer bytes in flight than the 16-byte staging loads).
+DIRECT_LOAD_MIN_UNITARIES = 32
+
+
 class ProgramKernels:
     """The specialised kernels of every pass of one encoded forward program.
 
@@ -1446,7 +1500,8 @@ class ProgramKernels:
                  prefetch: bool = False, waves: int = 2, readout=None, valu_pairs: int = 0,
                  grid: str = "full", pipeline: int = 0, lanes="lds", split_pairs: bool = True,
                  interleave: int = 4, wave_sync: bool = True, stagger: int = 0, load: bool = True,
-                 async_loads: bool = False, tail_prefetch: bool = False):
+                 async_loads: bool = False, tail_prefetch: bool = False, direct_store: bool = False,
+                 direct_load=DIRECT_LOAD_MIN_UNITARIES):
         self.n, self.R, self.TB = n, R, TB
         self.pipelined = bool(pipeline) and n >= R + TB
         self.block = (1 << TB) + ((1 << pipeline) if self.pipelined else 0)
@@ -1461,6 +1516,9 @@ class ProgramKernels:
             lane_off = len(self.lane_index)
             if lanes == "lds" and not self.pipelined:
                 self.lane_index += lane_entries(words, R, TB, n < R + TB)
+            # direct_load: True, False, or the fewest fused unitaries a pass needs to load directly
+            units = sum(len(bits) for sub in PL.decode_pass(words, R, TB) for bits, _cn in sub.segments)
+            dload = direct_load if isinstance(direct_load, bool) else (0 < direct_load <= units)
             use_async = async_loads and not self.pipelined and n >= R + TB and not (gen_first and i == 0)
             use_tail = tail_prefetch and not use_async and not self.pipelined and n >= R + TB
             self.persistent.append(use_async or use_tail)
@@ -1477,7 +1535,8 @@ class ProgramKernels:
             else:
                 self.sources.append(pass_source(nm, words, n, R, TB, lpos, gen_first and i == 0,
                                                 measure and i == npass - 1, prefetch, waves, readout, valu_pairs,
-                                                lanes, split_pairs, lane_off, interleave, wave_sync, stagger))
+                                                lanes, split_pairs, lane_off, interleave, wave_sync, stagger,
+                                                direct_store=direct_store, direct_load=dload))
             self.names.append(nm)
         if load:
             self.funcs = kernels_for(self.sources, self.names)

@@ -78,7 +78,7 @@ class UnevenBatch:
                  cost: PL.CostModel = None):
         self.device = torch.device(device)
         self.R, self.TB, self.grid = R, TB, grid
-        self.cost = cost or fused.INTERPRETED_COST
+        self.cost = cost or fused.UNEVEN_COST
         lib = L.load()
         self.lib = lib
         self.circs: typing.List[_Circ] = []

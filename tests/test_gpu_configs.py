@@ -92,7 +92,7 @@ def test_c4_full_mix(device, R, TB):
         assert np.array_equal(ws[i].inputs["x"].numpy()[0], fx[f"x{i}"])
         assert np.array_equal(_thetas(ws[i]), fx[f"theta{i}"])
         assert np.allclose(lw[i].double().numpy()[0], fx[f"w{i}"])
-    ub = UnevenBatch.from_workloads(ws, device, R=R, TB=TB, cost=fused.PL.CostModel(budget=160.0, min_low=4))
+    ub = UnevenBatch.from_workloads(ws, device, R=R, TB=TB, cost=fused.UNEVEN_COST)
     ub.set_loss_weights(lw)
     ub.capture(backward=True)
     for rep in range(2):

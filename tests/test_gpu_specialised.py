@@ -189,14 +189,16 @@ def test_specialised_folded_readout_and_gradients(device, specialise, n, B):
 @pytest.mark.parametrize("opts", [{"pipeline": 8}, {"lanes": True}, {"lanes": False, "split_pairs": False},
                                   {"grid": "resident", "prefetch": True}, {"interleave": False},
                                   {"interleave": 2, "wave_sync": False}, {"stagger": 2},
-                                  {"async_loads": True}, {"tail_prefetch": True}])
+                                  {"async_loads": True}, {"tail_prefetch": True}, {"direct_store": True},
+                                  {"direct_load": True, "direct_store": True}])
 def test_pass_kernel_variants_are_bit_identical(device, specialise, opts):
     """The pass-kernel variants (pipelined load/compute roles, operand loads,
     scalar operands with 16-byte tile moves, a resident grid with prefetch,
     one or two MFMA chains instead of four, block barriers between every
     sub-pass, a delayed start, direct global->LDS tile loads into two buffers,
-    the next tile streamed in during the last sub-pass with direct stores)
-    compute exactly what the default kernels do."""
+    the next tile streamed in during the last sub-pass with direct stores,
+    registers loaded from and stored to the state directly) compute exactly
+    what the default kernels do."""
     from qandle_amd._engine import fused
 
     n, B = 16, 3

@@ -251,12 +251,15 @@ def test_async_pass_sources_compile(tmp_path):
     assert ops and waits and blocks >= 2
 
 
-def test_tail_direct_stores_match_the_tile_layout():
-    """In the tile-streaming pass kernels the last sub-pass stores register j of
-    thread tq to gbase | sum of 1 << GP[r] over j's bits, where the tile write-back
-    would put it at tile element gl ^ (xor of 1 << P[r]) and from there at
-    scatter(element): for every pass of a 20-qubit program the two addresses agree,
-    and over all threads and registers each tile amplitude is stored exactly once."""
+def test_direct_register_loads_and_stores_match_the_tile_layout():
+    """Kernels that load the first sub-pass's registers straight from the state and
+    store the last sub-pass's straight to it use, for register j of thread tq, the
+    amplitude gbase | sum of 1 << GP[r] over j's bits (jit._register_offsets), where
+    the tile path would use tile element gl ^ (xor of 1 << P[r]) and from there
+    scatter(element): for the first and last sub-pass of every pass of a 20-qubit
+    program the two addresses agree, and over all threads and registers each tile
+    amplitude is moved exactly once."""
+    from qandle_amd._engine import jit
     from qandle_amd import workloads
     from qandle_amd._engine import fused
 
@@ -272,21 +275,22 @@ def test_tail_direct_stores_match_the_tile_layout():
     for wd, _ns, lpos in ws.passes:
         lpos = [int(x) for x in lpos]
         scat = lambda l: sum(1 << lpos[k] for k in range(K) if (l >> k) & 1)  # noqa: E731
-        sub = PL.decode_pass(wd.numpy(), R, TB)[-1]
-        seen = []
-        for tq in range(1 << TB):
-            gl = sum(((tq >> i) & 1) << sub.G[i] for i in range(TB))
-            gbase = sum(((tq >> i) & 1) << sub.GG[i] for i in range(TB))
-            for j in range(1 << R):
-                el = gl
-                off = gbase
-                for r in range(R):
-                    if (j >> r) & 1:
-                        el ^= 1 << sub.P[r]
-                        off |= 1 << sub.GP[r]
-                assert off == scat(el)
-                seen.append(off)
-        assert sorted(seen) == sorted(scat(l) for l in range(1 << K))
-        # lane bits: the lowest tile bits that are neither registers nor wave bits, ascending
-        taken = set(sub.P[:R]) | set(sub.G[6:TB])
-        assert [sub.G[i] for i in range(6)] == [b for b in range(K) if b not in taken][:6]
+        decoded = PL.decode_pass(wd.numpy(), R, TB)
+        for sub in (decoded[0], decoded[-1]):
+            roff, r0 = jit._register_offsets(sub, R)
+            assert r0 is None or sub.GP[r0] == 0
+            seen = []
+            for tq in range(1 << TB):
+                gl = sum(((tq >> i) & 1) << sub.G[i] for i in range(TB))
+                gbase = sum(((tq >> i) & 1) << sub.GG[i] for i in range(TB))
+                for j in range(1 << R):
+                    el = gl
+                    for r in range(R):
+                        if (j >> r) & 1:
+                            el ^= 1 << sub.P[r]
+                    assert gbase & roff[j] == 0 and gbase | roff[j] == scat(el)
+                    seen.append(gbase | roff[j])
+            assert sorted(seen) == sorted(scat(l) for l in range(1 << K))
+            # lane bits: the lowest tile bits that are neither registers nor wave bits, ascending
+            taken = set(sub.P[:R]) | set(sub.G[6:TB])
+            assert [sub.G[i] for i in range(6)] == [b for b in range(K) if b not in taken][:6]
