@@ -922,6 +922,15 @@ def _register_offsets(sub, R):
     return roff, next((r for r in range(R) if sub.GP[r] == 0), None)
 
 
+def _full_lines(sub, R) -> bool:
+    """Whether one wave's direct register loads or stores of a sub-pass fill whole
+    128-byte lines: the lowest lane bits sit on amplitude bits 0..3 (8-byte accesses)
+    or, with a register on amplitude bit 0, on bits 1..3 (16-byte accesses)."""
+    _roff, r0 = _register_offsets(sub, R)
+    low = [1, 2, 3] if r0 is not None else [0, 1, 2, 3]
+    return [sub.GG[b] for b in range(len(low))] == low
+
+
 def _direct_load(e, sub, R):
     """Load a sub-pass's registers straight from the state: register j of a thread
     is amplitude gbase | sum of 1 << GP[r] over its set bits r (the lane bits are the
@@ -1478,6 +1487,12 @@ def lane_mode(lanes, pipelined: bool):
 # results bit for bit the same), while streaming-bound passes lose by it (257 vs 197 us for
 # 12 unitaries: its 8-byte loads keep fewer bytes in flight than the 16-byte staging loads).
 DIRECT_LOAD_MIN_UNITARIES = 32
+# "auto": the last sub-pass stores its registers straight to the state in the generating
+# first pass (5-6% faster: the tile never goes back through LDS), the measuring last pass
+# (4-8%: it skips the LDS write-back) and any pass whose direct stores fill whole 128-byte
+# lines (3-5%); elsewhere a wave's direct stores scatter over up to 64 lines and cost up to 33%
+# (24 qubits), so those passes store through the tile.  Results are bit for bit the same.
+DIRECT_STORE = "auto"
 
 
 class ProgramKernels:
@@ -1500,7 +1515,7 @@ class ProgramKernels:
                  prefetch: bool = False, waves: int = 2, readout=None, valu_pairs: int = 0,
                  grid: str = "full", pipeline: int = 0, lanes="lds", split_pairs: bool = True,
                  interleave: int = 4, wave_sync: bool = True, stagger: int = 0, load: bool = True,
-                 async_loads: bool = False, tail_prefetch: bool = False, direct_store: bool = False,
+                 async_loads: bool = False, tail_prefetch: bool = False, direct_store=DIRECT_STORE,
                  direct_load=DIRECT_LOAD_MIN_UNITARIES):
         self.n, self.R, self.TB = n, R, TB
         self.pipelined = bool(pipeline) and n >= R + TB
@@ -1516,9 +1531,14 @@ class ProgramKernels:
             lane_off = len(self.lane_index)
             if lanes == "lds" and not self.pipelined:
                 self.lane_index += lane_entries(words, R, TB, n < R + TB)
-            # direct_load: True, False, or the fewest fused unitaries a pass needs to load directly
-            units = sum(len(bits) for sub in PL.decode_pass(words, R, TB) for bits, _cn in sub.segments)
+            # direct_load: True, False, or the fewest fused unitaries a pass needs to load directly;
+            # direct_store: True, False or "auto" (DIRECT_STORE_AUTO)
+            subs_i = PL.decode_pass(words, R, TB)
+            units = sum(len(bits) for sub in subs_i for bits, _cn in sub.segments)
             dload = direct_load if isinstance(direct_load, bool) else (0 < direct_load <= units)
+            dstore = direct_store
+            if direct_store == "auto":
+                dstore = (gen_first and i == 0) or (measure and i == npass - 1) or _full_lines(subs_i[-1], R)
             use_async = async_loads and not self.pipelined and n >= R + TB and not (gen_first and i == 0)
             use_tail = tail_prefetch and not use_async and not self.pipelined and n >= R + TB
             self.persistent.append(use_async or use_tail)
@@ -1536,7 +1556,7 @@ class ProgramKernels:
                 self.sources.append(pass_source(nm, words, n, R, TB, lpos, gen_first and i == 0,
                                                 measure and i == npass - 1, prefetch, waves, readout, valu_pairs,
                                                 lanes, split_pairs, lane_off, interleave, wave_sync, stagger,
-                                                direct_store=direct_store, direct_load=dload))
+                                                direct_store=dstore, direct_load=dload))
             self.names.append(nm)
         if load:
             self.funcs = kernels_for(self.sources, self.names)
