@@ -140,7 +140,7 @@ def profile_workload(mode, reps=4):
     circ = qandle.Circuit(w.layers, num_qubits=w.n).to(dev)
     x = w.inputs["x"].to(dev)
     xg = x.clone().requires_grad_(True)
-    lw = w.loss_w.to(dev)
+    lw = None if w.loss_w is None else w.loss_w.to(dev)
     params = list(circ.parameters())
 
     def step():

@@ -1079,7 +1079,8 @@ def adjoint_lane_entries(words, R: int, TB: int):
 def adjoint_source(name: str, words, n: int, R: int, TB: int, lpos, seed: bool,
                    prefetch: bool = False, waves: int = None, readout=None, lanes=False,
                    split_pairs: bool = False, lane_off: int = 0, interleave: bool = False,
-                   wave_sync: bool = False, hermitian: bool = False) -> str:
+                   wave_sync: bool = False, hermitian: bool = False, direct_load: bool = False,
+                   direct_store: bool = False) -> str:
     """HIP source of one specialised adjoint pass kernel (the adjoint tile of
     qs_adjoint_pass with the pass program compiled in).  ``seed``: the first
     step of the sweep, whose co-state is built from psi and w = dL/dP, with
@@ -1090,10 +1091,17 @@ def adjoint_source(name: str, words, n: int, R: int, TB: int, lpos, seed: bool,
     gradient of every fused rotation reads Im Tr(Q M) through the four real
     combinations of M that a Hermitian generator Q weighs (herm_trace), 12
     fused multiply-adds per amplitude pair instead of 16 (and of 24 separate
-    products and sums); equal to float rounding."""
+    products and sums); equal to float rounding.  ``direct_load`` /
+    ``direct_store``: the first sub-pass reads psi and lambda straight from the
+    state and the last writes them straight back, instead of staging the tiles
+    through LDS (tiles within one sample; direct_load not for the seeding step,
+    which builds lambda while staging, nor with prefetch); same results bit for
+    bit."""
     K = R + TB
     T, NR = 1 << TB, 1 << R
     multi = n < K
+    direct_load = direct_load and not multi and not seed and not prefetch
+    direct_store = direct_store and not multi
     subs = PL.decode_adjoint_pass(words, R, TB)
     lpos = [int(x) for x in lpos]
     npos = [] if multi else [p for p in range(n) if p not in lpos]
@@ -1149,14 +1157,14 @@ def adjoint_source(name: str, words, n: int, R: int, TB: int, lpos, seed: bool,
         e("  if (t < A.ntiles) fetch(tile_base(t));")
     e("  for (; t < A.ntiles; t += gridDim.x) {")
     e("    const i64 base = tile_base(t);")
-    if not prefetch:
+    if not prefetch and not direct_load:
         e("    fetch(base);")
     e(f"    const i64 block_sample = base >> {n};")
     e("    const int ts = pinv(tid);")
     if lane_k:
         nl = 4 * len(lane_k)
         e(f"    for (int c = ts; c < {nl}; c += {T}) ctab[c] = reinterpret_cast<const float4 *>(A.ltab)[{4 * lane_off} + c];")
-    for it in range(NR // 2):
+    for it in range(0 if direct_load else NR // 2):
         e(f"    {{ const int sl = swz(2 * ({it * T} + ts)); const bool sw = sl & 1; const float4 a = pa[{it}];")
         if seed:
             # co-state of MeasureProbability for the pair (gi, gi + 1)
@@ -1196,7 +1204,13 @@ def adjoint_source(name: str, words, n: int, R: int, TB: int, lpos, seed: bool,
     e(f"    float2 vp[{NR}], vl[{NR}];")
     for s, sub in enumerate(subs):
         _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog, lane_k, interleave,
-                      wave_sync and s + 1 < len(subs) and wave_local(sub, subs[s + 1], TB), hermitian)
+                      wave_sync and s + 1 < len(subs) and wave_local(sub, subs[s + 1], TB), hermitian,
+                      direct_load=direct_load and s == 0, direct_store=direct_store and s == len(subs) - 1)
+    if direct_store:
+        e("    __syncthreads();")
+        e("  }")
+        e("}")
+        return "\n".join(o)
     e("    const int tz = pinv(tid);")
     for it in range(NR // 2):
         e(f"    {{ const i64 gi = base | g_tid | {goff[it]}LL; const int sl = swz(2 * ({it * T} + tz)); const bool sw = sl & 1;")
@@ -1248,7 +1262,10 @@ def _emit_put(e, expr: str, row: int, wlog: int, indent: str):
 
 
 def _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog, lanes=None, interleave=False, wave_sync=False,
-                  hermitian=False):
+                  hermitian=False, direct_load=False, direct_store=False):
+    """One adjoint sub-pass.  ``direct_load`` (the first sub-pass): psi and lambda
+    come straight from the state (psi from ``A.psi_src``) instead of the tiles;
+    ``direct_store`` (the last): they go straight back to the state."""
     NR = 1 << R
     e(f"    {{  // adjoint sub-pass {s}")
     e("      const int tq = pinv(tid);")
@@ -1267,8 +1284,26 @@ def _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog, lanes=None, interleave=False
             if (j >> r) & 1:
                 a ^= sw[r]
         swj.append(a)
-    for j in range(NR):
-        e(f"      vp[{j}] = lds_at(tp, lb8 ^ {8 * swj[j]}); vl[{j}] = lds_at(tl, lb8 ^ {8 * swj[j]});")
+    if direct_load:
+        roff, r0 = _register_offsets(sub, R)
+        e("      {")
+        for j in range(NR):
+            if r0 is not None:
+                if (j >> r0) & 1:
+                    continue
+                k = j | (1 << r0)
+                e(f"        {{ const i64 q = (gbase | {roff[j]}LL) >> 1;")
+                e("          const float4 a = reinterpret_cast<const float4 *>(A.psi_src)[q], "
+                  "b = reinterpret_cast<const float4 *>(A.lam)[q];")
+                e(f"          vp[{j}] = make_float2(a.x, a.y); vp[{k}] = make_float2(a.z, a.w); "
+                  f"vl[{j}] = make_float2(b.x, b.y); vl[{k}] = make_float2(b.z, b.w); }}")
+            else:
+                e(f"        vp[{j}] = reinterpret_cast<const float2 *>(A.psi_src)[gbase | {roff[j]}LL]; "
+                  f"vl[{j}] = reinterpret_cast<const float2 *>(A.lam)[gbase | {roff[j]}LL];")
+        e("      }")
+    else:
+        for j in range(NR):
+            e(f"      vp[{j}] = lds_at(tp, lb8 ^ {8 * swj[j]}); vl[{j}] = lds_at(tl, lb8 ^ {8 * swj[j]});")
     cq = {}
     if lanes:
         for si_, (_cn, bits) in enumerate(sub.segments):
@@ -1372,6 +1407,21 @@ def _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog, lanes=None, interleave=False
                             a, b = phys[j], phys[j | (1 << r)]
                             e(f"        mfma_pair(vp[{a}], vp[{b}], cf); mfma_pair(vl[{a}], vl[{b}], cf);")
             e("      }")
+    if direct_store:
+        roff, r0 = _register_offsets(sub, R)
+        for j in range(NR):
+            if r0 is not None:
+                if (j >> r0) & 1:
+                    continue
+                a, b = phys[j], phys[j | (1 << r0)]
+                e(f"      {{ const i64 q = (gbase | {roff[j]}LL) >> 1;")
+                e(f"        reinterpret_cast<float4 *>(A.psi)[q] = make_float4(vp[{a}].x, vp[{a}].y, vp[{b}].x, vp[{b}].y);")
+                e(f"        reinterpret_cast<float4 *>(A.lam)[q] = make_float4(vl[{a}].x, vl[{a}].y, vl[{b}].x, vl[{b}].y); }}")
+            else:
+                e(f"      reinterpret_cast<float2 *>(A.psi)[gbase | {roff[j]}LL] = vp[{phys[j]}]; "
+                  f"reinterpret_cast<float2 *>(A.lam)[gbase | {roff[j]}LL] = vl[{phys[j]}];")
+        e("    }")
+        return
     for j in range(NR):
         e(f"      lds_at(tp, lb8 ^ {8 * swj[j]}) = vp[{phys[j]}]; lds_at(tl, lb8 ^ {8 * swj[j]}) = vl[{phys[j]}];")
     if wave_sync:  # the next step keeps the wave-index bits (wave_local)
@@ -1606,7 +1656,8 @@ class AdjointKernels:
 
     def __init__(self, passes, n: int, R: int, TB: int, seed_first: bool, prefetch: bool = False,
                  waves: int = 2, readout=None, grid: str = "full", lanes="lds", split_pairs: bool = True,
-                 interleave: bool = True, wave_sync: bool = True, hermitian: bool = True):
+                 interleave: bool = True, wave_sync: bool = True, hermitian: bool = True,
+                 direct_load=False, direct_store=False):
         self.block = 1 << TB
         self.full_grid = grid == "full"
         self.sources, self.names = [], []
@@ -1617,9 +1668,15 @@ class AdjointKernels:
             lane_off = len(self.lane_index)
             if lanes == "lds":
                 self.lane_index += adjoint_lane_entries(words, R, TB)
+            # direct_load: True, False or the fewest fused unitaries a step needs to load directly;
+            # direct_store: True, False or "auto" (whole 128-byte lines, as for the forward)
+            subs_i = PL.decode_adjoint_pass(words, R, TB)
+            units = sum(len(bits) for sub in subs_i for _cn, bits in sub.segments)
+            dload = direct_load if isinstance(direct_load, bool) else (0 < direct_load <= units)
+            dstore = _full_lines(subs_i[-1], R) if direct_store == "auto" else direct_store
             self.sources.append(adjoint_source(nm, words, n, R, TB, lpos, seed_first and i == 0, prefetch, waves,
                                                readout, lanes, split_pairs, lane_off, interleave, wave_sync,
-                                               hermitian))
+                                               hermitian, direct_load=dload, direct_store=dstore))
             self.names.append(nm)
         self.funcs = kernels_for(self.sources, self.names)
         self.grid_cap = [None] * len(self.funcs)

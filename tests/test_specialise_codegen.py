@@ -256,26 +256,34 @@ def test_direct_register_loads_and_stores_match_the_tile_layout():
     store the last sub-pass's straight to it use, for register j of thread tq, the
     amplitude gbase | sum of 1 << GP[r] over j's bits (jit._register_offsets), where
     the tile path would use tile element gl ^ (xor of 1 << P[r]) and from there
-    scatter(element): for the first and last sub-pass of every pass of a 20-qubit
-    program the two addresses agree, and over all threads and registers each tile
-    amplitude is moved exactly once."""
+    scatter(element): for the first and last sub-pass of every forward pass and
+    adjoint step of a 20-qubit program the two addresses agree, and over all threads
+    and registers each tile amplitude is moved exactly once."""
     from qandle_amd._engine import jit
     from qandle_amd import workloads
     from qandle_amd._engine import fused
 
-    n, R, TB = 20, 5, 8
-    K = R + TB
+    n = 20
     w = workloads.rxryrz_ladder(n=n, B=2, depth=4, seed=3)
     circ = qandle.Circuit(w.layers, num_qubits=n)
     prog = fused.lower_circuit(circ.circuit)
     lead = fused.leading_rotations(prog)
     p = lead[0] if lead else prog
+    cases = []
+    R, TB = 5, 8
     pl = fused.get_plan(p, R, TB, None, True)
-    ws = fused.pass_words(p, pl, 2, "cpu")
+    for wd, _ns, lpos in fused.pass_words(p, pl, 2, "cpu").passes:
+        cases.append((R, TB, lpos, PL.decode_pass(wd.numpy(), R, TB)))
+    # the adjoint steps (gradient tile shape) use the same register layout
+    R, TB = fused.choose_shape(n, True)
+    pl = fused.get_plan(prog, R, TB, fused.GRAD_COST, True)
+    ws, _nmain = fused.adjoint_words(prog, pl, 2, "cpu", True, fused.GRAD_COST)
     for wd, _ns, lpos in ws.passes:
+        cases.append((R, TB, lpos, PL.decode_adjoint_pass(wd.numpy(), R, TB)))
+    for R, TB, lpos, decoded in cases:
+        K = R + TB
         lpos = [int(x) for x in lpos]
         scat = lambda l: sum(1 << lpos[k] for k in range(K) if (l >> k) & 1)  # noqa: E731
-        decoded = PL.decode_pass(wd.numpy(), R, TB)
         for sub in (decoded[0], decoded[-1]):
             roff, r0 = jit._register_offsets(sub, R)
             assert r0 is None or sub.GP[r0] == 0
