@@ -1393,9 +1393,18 @@ def _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog, lanes=None, interleave=False
                 elif (si_, r) in cq:
                     nm = cq[(si_, r)]
                     e(f"        const float cf[4] = {{{nm}.x, {nm}.y, {nm}.z, {nm}.w}};")
-                    for j in range(NR):
-                        if not (j >> r) & 1:
-                            a, b = phys[j], phys[j | (1 << r)]
+                    prs = [(phys[j], phys[j | (1 << r)]) for j in range(NR) if not (j >> r) & 1]
+                    if interleave == 4:
+                        # the psi and lambda pairs of two amplitude pairs: four chains interleaved
+                        for k in range(0, len(prs) - 1, 2):
+                            (a, b), (c, d) = prs[k], prs[k + 1]
+                            e(f"        mfma_pair4(vp[{a}], vp[{b}], vl[{a}], vl[{b}], vp[{c}], vp[{d}], "
+                              f"vl[{c}], vl[{d}], cf);")
+                        if len(prs) % 2:
+                            a, b = prs[-1]
+                            e(f"        mfma_pair2(vp[{a}], vp[{b}], vl[{a}], vl[{b}], cf);")
+                    else:
+                        for a, b in prs:
                             if interleave:
                                 e(f"        mfma_pair2(vp[{a}], vp[{b}], vl[{a}], vl[{b}], cf);")
                             else:
