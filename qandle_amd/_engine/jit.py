@@ -1662,7 +1662,9 @@ class AdjointKernels:
     step of 32+ fused unitaries loads psi and lambda straight from the state; a
     step whose last sub-pass's stores fill whole 128-byte lines stores them
     straight back): 4.07-4.08 vs 4.17-4.20 ms for the 15 steps at 16 qubits x
-    256 x 20 layers, gradients bit for bit the same.
+    256 x 20 layers, gradients bit for bit the same.  ``interleave`` = 4 (two
+    amplitude pairs' psi and lambda chains at a time) gives the same bits and
+    measured 1.5% slower than the default two chains (4.21-4.24 vs 4.15-4.16 ms).
 
     Defaults as for ProgramKernels (two waves per SIMD, the U^dagger lane
     table in LDS, 8-byte pair moves): 18.1 vs 22.2 ms per forward + backward

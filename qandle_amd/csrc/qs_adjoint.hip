@@ -263,21 +263,21 @@ __device__ __forceinline__ void adjoint_finish_row(const AdjFinishArgs &F, int64
   if (F.sps <= 64) {
     const int64_t total = F.B * F.sps;
     const int seg = (int)F.sps;
-    // four chunks (wave, wave + 4, ... of 64 slots) per step, their loads in flight together
+    // eight chunks (wave, wave + 4, ... of 64 slots) per step, their loads in flight together
     // before their shuffle sums
-    for (int64_t c0 = (int64_t)wave * 64; c0 < total; c0 += 1024) {
-      double acc[4];
+    for (int64_t c0 = (int64_t)wave * 64; c0 < total; c0 += 2048) {
+      double acc[8];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < 8; ++u) {
         const int64_t i = c0 + u * 256 + lane;
         acc[u] = i < total ? (double)g[i] : 0.0;
       }
       for (int off = 1; off < seg; off <<= 1) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc[u] += __shfl_xor(acc[u], off, 64);
+        for (int u = 0; u < 8; ++u) acc[u] += __shfl_xor(acc[u], off, 64);
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < 8; ++u) {
         const int64_t i = c0 + u * 256 + lane;
         if ((lane & (seg - 1)) == 0 && i < total) {
           F.out[row * F.B + i / seg] = (float)(0.5 * acc[u]);
