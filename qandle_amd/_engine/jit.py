@@ -195,6 +195,32 @@ __device__ __forceinline__ float half_sum(float s) {
   const float lo = lane_f(r, 0) + lane_f(r, 16), hi = lane_f(r, 32) + lane_f(r, 48);
   return (threadIdx.x & 32) ? hi : lo;
 }
+// Lane l + 16 (row swap, permlane16_swap) and lane l + 32 (half swap, permlane32_swap) folded in:
+// with the same value as both operands, the two results of a swap are the value's two halves, so
+// their sum is s[l] + s[l ^ 16] (s[l] + s[l ^ 32]) in every lane.
+__device__ __forceinline__ float swap16_add(float s) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(s), __float_as_uint(s), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float swap32_add(float s) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(s), __float_as_uint(s), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+// Four values summed over a 64-lane wave together: two exchange rounds (lane l ^ 1, then l ^ 2) leave
+// each lane with one value's quad sum, value index 2 * (l & 1) + ((l >> 1) & 1); row_ror 4 and 8 add
+// the row's lanes of the same index, the two swaps the four rows.  Every lane ends with the wave sum
+// of its value index (lanes 0, 2, 1, 3 hold values 0, 1, 2, 3).
+__device__ __forceinline__ float wave_sum4(float t0, float t1, float t2, float t3) {
+  const int l = threadIdx.x;
+  const bool b0 = l & 1, b1 = l & 2;
+  const float k0 = b0 ? t2 : t0, k1 = b0 ? t3 : t1, s0 = b0 ? t0 : t2, s1 = b0 ? t1 : t3;
+  const float u0 = k0 + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(s0), 0xB1, 0xF, 0xF, false));
+  const float u1 = k1 + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(s1), 0xB1, 0xF, 0xF, false));
+  const float k = b1 ? u1 : u0, s = b1 ? u0 : u1;
+  float w = k + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(s), 0x4E, 0xF, 0xF, false));
+  w = dpp_add<0x128>(dpp_add<0x124>(w));
+  return swap32_add(swap16_add(w));
+}
 struct JAdj {
   float *psi; float *lam; const int *utab; const int *qtab; const int *cs; float *gpart; const float *seed_w;
   const float *ltab;
@@ -1080,7 +1106,7 @@ def adjoint_source(name: str, words, n: int, R: int, TB: int, lpos, seed: bool,
                    prefetch: bool = False, waves: int = None, readout=None, lanes=False,
                    split_pairs: bool = False, lane_off: int = 0, interleave: bool = False,
                    wave_sync: bool = False, hermitian: bool = False, direct_load: bool = False,
-                   direct_store: bool = False) -> str:
+                   direct_store: bool = False, rows4: bool = True) -> str:
     """HIP source of one specialised adjoint pass kernel (the adjoint tile of
     qs_adjoint_pass with the pass program compiled in).  ``seed``: the first
     step of the sweep, whose co-state is built from psi and w = dL/dP, with
@@ -1096,7 +1122,10 @@ def adjoint_source(name: str, words, n: int, R: int, TB: int, lpos, seed: bool,
     state and the last writes them straight back, instead of staging the tiles
     through LDS (tiles within one sample; direct_load not for the seeding step,
     which builds lambda while staging, nor with prefetch); same results bit for
-    bit."""
+    bit.  ``rows4``: a unitary's gradient rows are summed over the wave four at
+    a time (wave_sum4: two exchange rounds, row rotations and the row and half
+    swaps, the sums landing in lanes 0-3 for one store) instead of one DPP and
+    readlane reduction per row; float sums in another order."""
     K = R + TB
     T, NR = 1 << TB, 1 << R
     multi = n < K
@@ -1205,7 +1234,8 @@ def adjoint_source(name: str, words, n: int, R: int, TB: int, lpos, seed: bool,
     for s, sub in enumerate(subs):
         _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog, lane_k, interleave,
                       wave_sync and s + 1 < len(subs) and wave_local(sub, subs[s + 1], TB), hermitian,
-                      direct_load=direct_load and s == 0, direct_store=direct_store and s == len(subs) - 1)
+                      direct_load=direct_load and s == 0, direct_store=direct_store and s == len(subs) - 1,
+                      rows4=rows4)
     if direct_store:
         e("    __syncthreads();")
         e("  }")
@@ -1262,7 +1292,7 @@ def _emit_put(e, expr: str, row: int, wlog: int, indent: str):
 
 
 def _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog, lanes=None, interleave=False, wave_sync=False,
-                  hermitian=False, direct_load=False, direct_store=False):
+                  hermitian=False, direct_load=False, direct_store=False, rows4=True):
     """One adjoint sub-pass.  ``direct_load`` (the first sub-pass): psi and lambda
     come straight from the state (psi from ``A.psi_src``) instead of the tiles;
     ``direct_store`` (the last): they go straight back to the state."""
@@ -1378,10 +1408,26 @@ def _adj_sub_pass(e, sub, s, n, R, TB, multi, wlog, lanes=None, interleave=False
             var = "csample" if ps else "0"
             trace = "herm_trace" if hermitian else "im_trace"
             acc = "h" if hermitian else "m"
-            for k, row in enumerate(rows):
-                if row < 0:
-                    continue
-                _emit_put(e, f"{trace}(qt + 8 * ({q} + {k} * {span} + {var}), {acc})", row, wlog, "        ")
+            live = [(k, row) for k, row in enumerate(rows) if row >= 0]
+            if wlog == 6 and rows4:
+                # the rows' per-lane traces summed over the wave four at a time (wave_sum4); lanes
+                # 0..3 then hold them and store them with one instruction
+                for c0 in range(0, len(live), 4):
+                    grp = live[c0:c0 + 4]
+                    ts = [f"{trace}(qt + 8 * ({q} + {k} * {span} + {var}), {acc})" for k, _row in grp]
+                    ts += ["0.f"] * (4 - len(grp))
+                    rws = [row for _k, row in grp] + [-1] * (4 - len(grp))
+                    e(f"        {{ const float g4 = wave_sum4({', '.join(ts)});")
+                    e("          const int ln = tid & 63;")
+                    e("          if (ln < 4) {")
+                    e("            const int v = 2 * (ln & 1) + ((ln >> 1) & 1);")
+                    e(f"            const int row = v == 0 ? {rws[0]} : v == 1 ? {rws[1]} : v == 2 ? {rws[2]} : {rws[3]};")
+                    e("            if (row >= 0) pinf(A.gpart)[(i64)row * A.NS + uslot] = g4;")
+                    e("          }")
+                    e("        }")
+            else:
+                for k, row in live:
+                    _emit_put(e, f"{trace}(qt + 8 * ({q} + {k} * {span} + {var}), {acc})", row, wlog, "        ")
             if not probe:
                 off = f"8 * ({u} + csample)" if ps else f"{8 * u}"
                 if ps and multi:
@@ -1674,7 +1720,7 @@ class AdjointKernels:
     def __init__(self, passes, n: int, R: int, TB: int, seed_first: bool, prefetch: bool = False,
                  waves: int = 2, readout=None, grid: str = "full", lanes="lds", split_pairs: bool = True,
                  interleave: bool = True, wave_sync: bool = True, hermitian: bool = True,
-                 direct_load=DIRECT_LOAD_MIN_UNITARIES, direct_store=DIRECT_STORE):
+                 direct_load=DIRECT_LOAD_MIN_UNITARIES, direct_store=DIRECT_STORE, rows4: bool = True):
         self.block = 1 << TB
         self.full_grid = grid == "full"
         self.sources, self.names = [], []
@@ -1693,7 +1739,7 @@ class AdjointKernels:
             dstore = _full_lines(subs_i[-1], R) if direct_store == "auto" else direct_store
             self.sources.append(adjoint_source(nm, words, n, R, TB, lpos, seed_first and i == 0, prefetch, waves,
                                                readout, lanes, split_pairs, lane_off, interleave, wave_sync,
-                                               hermitian, direct_load=dload, direct_store=dstore))
+                                               hermitian, direct_load=dload, direct_store=dstore, rows4=rows4))
             self.names.append(nm)
         self.funcs = kernels_for(self.sources, self.names)
         self.grid_cap = [None] * len(self.funcs)
