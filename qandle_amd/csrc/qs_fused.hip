@@ -178,15 +178,12 @@ __device__ __forceinline__ void pass_body(const PassArgs &A, const int64_t t, co
     for (int sg = 0; sg < nseg; ++sg) {
       const int head = op[0];
       const int umask = head & 0xff, ncnot = (head >> 8) & 0xff, psmask = (head >> 16) & 0xff;
-      // every table read of the segment is issued before the first use, and only for the
-      // register bits that have a unitary (umask is uniform: the branches are scalar)
+      // every table read of the segment is issued before the first use
       M8 u[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        if ((umask >> r) & 1) {
-          const int64_t ui = op[1 + r] + (((psmask >> r) & 1) ? csample : 0);
-          u[r] = load_m8(A.utab + 8 * ui);
-        }
+        const int64_t ui = op[1 + r] + (((psmask >> r) & 1) ? csample : 0);
+        u[r] = load_m8(A.utab + 8 * ui);
       }
 #pragma unroll
       for (int r = 0; r < R; ++r) {
