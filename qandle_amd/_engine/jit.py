@@ -1632,6 +1632,7 @@ class ProgramKernels:
         self.sources, self.names = [], []
         self.lane_index = []  # utab index of every lane-table entry (lanes == "lds"), pass by pass
         self.persistent = []  # per pass: True when its kernel walks tiles with a resident grid
+        self.direct = []  # per pass: (registers loaded from the state, registers stored to it)
         for i, (w, _nsub, lpos) in enumerate(passes):
             words = w.cpu().numpy() if hasattr(w, "cpu") else w
             nm = f"{KERNEL}_{i}"
@@ -1664,6 +1665,9 @@ class ProgramKernels:
                                                 measure and i == npass - 1, prefetch, waves, readout, valu_pairs,
                                                 lanes, split_pairs, lane_off, interleave, wave_sync, stagger,
                                                 direct_store=dstore, direct_load=dload))
+            plain, one = not (use_tail or use_async or self.pipelined), n >= R + TB  # pass_source's gating
+            self.direct.append((plain and one and bool(dload) and not (gen_first and i == 0) and not prefetch,
+                                plain and one and bool(dstore)))
             self.names.append(nm)
         if load:
             self.funcs = kernels_for(self.sources, self.names)

@@ -50,6 +50,46 @@ def test_specialised_ladder_vs_fp64(device, specialise, n, B, depth):
     assert (P1 - P0).abs().max().item() < 1e-6
 
 
+@pytest.mark.parametrize("n,B", [(10, 3), (11, 3), (12, 1), (13, 1), (13, 3), (14, 1), (14, 3), (21, 1)])
+def test_default_direct_paths_at_tile_boundary_vs_fp64(device, specialise, n, B):
+    """Widths around one tile per sample (a tile holds 2^11, 2^12 or 2^13
+    amplitudes for the shapes chosen at 10-11, 12 and 13-21 qubits; 10 qubits
+    packs two samples per tile), batches of 1 and 3: the default kernels, which
+    load a pass's registers straight from the state when it fuses 32 or more
+    unitaries and store them straight back where a wave writes whole lines,
+    match the FP64 oracle and, bit for bit, the kernels that stage every
+    register through the tile."""
+    from qandle_amd._engine import fused
+
+    w = workloads.rxryrz_ladder(n=n, B=B, depth=6, seed=300 + n + B)
+    circ = qandle.Circuit(w.layers, num_qubits=n).to(device)
+    x = w.inputs["x"].to(device)
+    config.SPECIALISE = True
+    old = config.SPECIALISE_OPTIONS
+    try:
+        config.SPECIALISE_OPTIONS = {}
+        with torch.no_grad():
+            P1 = circ(x=x).cpu()
+        R, TB = fused.choose_shape(n)
+        kern = fused.forward_kernels(fused.lower_circuit(circ.circuit), B, R, TB, device=device)
+        config.SPECIALISE_OPTIONS = {"direct_load": False, "direct_store": False}
+        with torch.no_grad():
+            P0 = circ(x=x).cpu()
+    finally:
+        config.SPECIALISE_OPTIONS = old
+    loads, stores = sum(a for a, _ in kern.direct), sum(b for _, b in kern.direct)
+    ref = fp64.measure_prob(fp64.simulate(n, w.oracle_ops()), n)
+    err = np.abs(P1.numpy() - ref).max()
+    print(f"n={n} B={B}: {len(kern.direct)} passes, {loads} load and {stores} store registers directly; "
+          f"max |P - P_fp64| = {err:.2e}")
+    if n >= R + TB:
+        assert stores >= 1 and (loads >= 1 or len(kern.direct) == 1)
+    else:
+        assert loads == stores == 0
+    assert err < ATOL
+    assert torch.equal(P0, P1)
+
+
 @pytest.mark.parametrize("n,B", [(6, 5), (12, 4)])
 def test_specialised_named_inputs_and_gradients(device, specialise, n, B):
     """Per-sample named rotations (the vector-ALU path in multi-sample tiles),
@@ -112,7 +152,7 @@ def test_specialised_state_in_state_out(device, specialise):
     assert np.abs(s_b.numpy() - ref).max() < ATOL
 
 
-@pytest.mark.parametrize("n,B,depth", [(8, 5, 3), (13, 3, 2), (16, 2, 2)])
+@pytest.mark.parametrize("n,B,depth", [(8, 5, 3), (12, 1, 3), (13, 3, 2), (14, 1, 4), (16, 2, 2)])
 def test_specialised_adjoint_vs_fp64(device, specialise, n, B, depth):
     """Specialised forward + specialised adjoint sweep against the FP64 adjoint (1e-4)."""
     w = workloads.rxryrz_ladder(n=n, B=B, depth=depth, seed=200 + n, loss=True)
