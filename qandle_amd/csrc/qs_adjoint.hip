@@ -38,23 +38,23 @@ struct AdjArgs {
   int32_t npos[QS_MAX_QUBITS];
 };
 
-template <int R, int TB, bool MULTI>
-__device__ __forceinline__ void adjoint_tile(const AdjArgs &A, const int64_t t, float2 *tp, float2 *tl) {
+template <int R, int TB, bool MULTI, class AP>
+__device__ __forceinline__ void adjoint_tile(AP A, const int64_t t, float2 *tp, float2 *tl) {
   constexpr int T = 1 << TB;
   constexpr int K = R + TB;
   constexpr int NR = 1 << R;
 
   const int tid = threadIdx.x;
-  const int n = A.n;
-  const int64_t total = A.B << n;
+  const int n = A->n;
+  const int64_t total = A->B << n;
   int64_t base;
   if (MULTI) {
     base = t << K;
   } else {
-    const int64_t tlo = t & ((int64_t(1) << A.nn) - 1);
-    base = (t >> A.nn) << n;
-    for (int i = 0; i < A.nn; ++i)
-      if ((tlo >> i) & 1) base |= int64_t(1) << A.npos[i];
+    const int64_t tlo = t & ((int64_t(1) << A->nn) - 1);
+    base = (t >> A->nn) << n;
+    for (int i = 0; i < A->nn; ++i)
+      if ((tlo >> i) & 1) base |= int64_t(1) << A->npos[i];
   }
   const int64_t block_sample = base >> n;
   const int seglog = MULTI ? (n - R) : TB;
@@ -65,21 +65,21 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &A, const int64_t t, 
   int64_t g_tid = 0;
 #pragma unroll
   for (int i = 0; i < TB; ++i)
-    if ((tid >> i) & 1) g_tid |= int64_t(1) << A.lpos[i + 1];
+    if ((tid >> i) & 1) g_tid |= int64_t(1) << A->lpos[i + 1];
 
 #pragma unroll
   for (int it = 0; it < (NR >> 1); ++it) {
     int64_t g_it = 0;
 #pragma unroll
     for (int i = 0; i < R - 1; ++i)
-      if ((it >> i) & 1) g_it |= int64_t(1) << A.lpos[TB + 1 + i];
+      if ((it >> i) & 1) g_it |= int64_t(1) << A->lpos[TB + 1 + i];
     const int64_t gi = base | g_tid | g_it;
     float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
     if (gi < total) {
-      a = reinterpret_cast<const float4 *>(A.psi)[gi >> 1];
-      if (A.seed_w) {
+      a = load_g4(A->psi, gi >> 1);
+      if (A->seed_w) {
         // co-state seed of MeasureProbability for the pair (gi, gi + 1)
-        const float *wb = A.seed_w + (gi >> n) * n;
+        const kptr<float> wb = to_const(A->seed_w) + (gi >> n) * n;
         float w0 = 0.f, w1 = 0.f;
         for (int pos = 1; pos < n; ++pos)
           if (!((gi >> pos) & 1)) w0 += wb[n - 1 - pos];
@@ -87,7 +87,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &A, const int64_t t, 
         w0 += wb[n - 1];  // position 0 is 0 for gi (even) and 1 for gi + 1
         b = make_float4(2.f * w0 * a.x, 2.f * w0 * a.y, 2.f * w1 * a.z, 2.f * w1 * a.w);
       } else {
-        b = reinterpret_cast<const float4 *>(A.lam)[gi >> 1];
+        b = load_g4(A->lam, gi >> 1);
       }
     }
     const int sl = swz(2 * (it * T + tid));
@@ -101,8 +101,8 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &A, const int64_t t, 
   __syncthreads();
 
   float2 vp[NR], vl[NR];
-  const int32_t *w = A.words;
-  for (int s = 0; s < A.nsub; ++s) {
+  kptr<int32_t> w = to_const(A->words);
+  for (int s = 0; s < A->nsub; ++s) {
     const int nseg = w[0], nphase = w[6];
     int gl = 0;
     int64_t gbase = base;
@@ -114,7 +114,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &A, const int64_t t, 
       }
     }
     const int64_t sample = MULTI ? (gbase >> n) : block_sample;
-    const int64_t csample = sample < A.B ? sample : 0;
+    const int64_t csample = sample < A->B ? sample : 0;
     const int lb = swz(gl);
     int sw[R];
 #pragma unroll
@@ -134,10 +134,10 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &A, const int64_t t, 
 
     auto reduce_put = [&](float g, int row) {
       for (int off = 1; off < (1 << wlog); off <<= 1) g += __shfl_xor(g, off, 64);
-      if (leader && row >= 0) A.gpart[(int64_t)row * A.NS + u] = g;
+      if (leader && row >= 0) to_global(A->gpart)[(int64_t)row * A->NS + u] = g;
     };
-    const int32_t *ph = w + kHdr;
-    const int32_t *op = ph + 4 * nphase;
+    const kptr<int32_t> ph = w + kHdr;
+    kptr<int32_t> op = ph + 4 * nphase;
     if (nphase) {
       // every phase entry commutes with the whole sub-pass: its gradient is read
       // at the sub-pass output, where Im<lam|Z_p|psi> = (+/-) sum Im(conj(lam) psi)
@@ -145,7 +145,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &A, const int64_t t, 
 #pragma unroll
       for (int j = 0; j < NR; ++j) S += vl[j].x * vp[j].y - vl[j].y * vp[j].x;
       for (int k = 0; k < nphase; ++k) reduce_put(((gbase >> ph[4 * k]) & 1) ? -S : S, ph[4 * k + 3]);
-      const float2 z = phase_product(ph, nphase, A.cs, csample, gbase, true);
+      const float2 z = phase_product(ph, nphase, to_const(A->cs), csample, gbase, true);
 #pragma unroll
       for (int j = 0; j < NR; ++j) {
         vp[j] = cmul(z, vp[j]);
@@ -168,12 +168,12 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &A, const int64_t t, 
           const int nconst = op[1];
           const bool ps = (psmask >> r) & 1;
           const int64_t var = ps ? csample : 0;
-          const int64_t span = ps ? A.B : 1;
+          const int64_t span = ps ? A->B : 1;
           const M8 m = cross_density_at<R>(vp, vl, r);
           for (int k = 0; k < nconst; ++k)
-            reduce_put(im_trace(load_m8(A.qtab + 8 * (op[2] + k * span + var)), m), op[3 + k]);
+            reduce_put(im_trace(load_m8(to_const(A->qtab) + 8 * (op[2] + k * span + var)), m), op[3 + k]);
           if (!((probemask >> r) & 1)) {
-            const M8 uu = load_m8(A.utab + 8 * (op[0] + var));
+            const M8 uu = load_m8(to_const(A->utab) + 8 * (op[0] + var));
             if (MULTI && ps) {  // may differ inside a 4-lane matrix block
               apply_valu_at<R>(vp, r, dagger(uu));
               apply_valu_at<R>(vl, r, dagger(uu));
@@ -202,7 +202,7 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &A, const int64_t t, 
     int64_t g_it = 0;
 #pragma unroll
     for (int i = 0; i < R - 1; ++i)
-      if ((it >> i) & 1) g_it |= int64_t(1) << A.lpos[TB + 1 + i];
+      if ((it >> i) & 1) g_it |= int64_t(1) << A->lpos[TB + 1 + i];
     const int64_t gi = base | g_tid | g_it;
     const int sl = swz(2 * (it * T + tid));
     float4 a = *reinterpret_cast<const float4 *>(&tp[sl & ~1]);
@@ -212,8 +212,8 @@ __device__ __forceinline__ void adjoint_tile(const AdjArgs &A, const int64_t t, 
       b = make_float4(b.z, b.w, b.x, b.y);
     }
     if (gi < total) {
-      reinterpret_cast<float4 *>(A.psi)[gi >> 1] = a;
-      reinterpret_cast<float4 *>(A.lam)[gi >> 1] = b;
+      store_g4(A->psi, gi >> 1, a);
+      store_g4(A->lam, gi >> 1, b);
     }
   }
 }
@@ -222,7 +222,7 @@ template <int R, int TB, bool MULTI>
 __global__ void __launch_bounds__(1 << TB) k_adjoint_pass(AdjArgs A) {
   __shared__ __align__(16) float2 tp[1 << (R + TB)];
   __shared__ __align__(16) float2 tl[1 << (R + TB)];
-  adjoint_tile<R, TB, MULTI>(A, blockIdx.x, tp, tl);
+  adjoint_tile<R, TB, MULTI>(&A, blockIdx.x, tp, tl);
 }
 
 template <int R, int TB>
@@ -230,10 +230,12 @@ __global__ void __launch_bounds__(1 << TB)
 k_adjoint_items(const AdjArgs *descs, const int32_t *items, int nitems) {
   __shared__ __align__(16) float2 tp[1 << (R + TB)];
   __shared__ __align__(16) float2 tl[1 << (R + TB)];
+  const kptr<AdjArgs> D = to_const(descs);
+  const kptr<int32_t> I = to_const(items);
   for (int i = blockIdx.x; i < nitems; i += gridDim.x) {
-    const AdjArgs &A = descs[items[2 * i]];
-    const int64_t t = items[2 * i + 1];
-    if (A.n < R + TB)
+    const kptr<AdjArgs> A = D + I[2 * i];
+    const int64_t t = I[2 * i + 1];
+    if (A->n < R + TB)
       adjoint_tile<R, TB, true>(A, t, tp, tl);
     else
       adjoint_tile<R, TB, false>(A, t, tp, tl);
@@ -455,6 +457,10 @@ int qs_items_adjoint(const void *descs, const int32_t *items, int nitems, int R,
   }
   if (R == 5 && TB == 6) {
     hipLaunchKernelGGL((k_adjoint_items<5, 6>), dim3(g), dim3(64), 0, st, d, items, nitems);
+    return launch_status();
+  }
+  if (R == 5 && TB == 8) {
+    hipLaunchKernelGGL((k_adjoint_items<5, 8>), dim3(g), dim3(256), 0, st, d, items, nitems);
     return launch_status();
   }
   return QS_EUNSUPPORTED;

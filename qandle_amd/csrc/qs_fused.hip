@@ -43,41 +43,42 @@ struct PassArgs {
 };
 
 // Global index of tile t's first amplitude (its non-local bits).
-__device__ __forceinline__ int64_t tile_base(const PassArgs &A, int64_t t, int K, bool multi) {
+template <class AP>
+__device__ __forceinline__ int64_t tile_base(AP A, int64_t t, int K, bool multi) {
   if (multi) return t << K;
-  const int64_t tl = t & ((int64_t(1) << A.nn) - 1);
-  int64_t base = (t >> A.nn) << A.n;
-  for (int i = 0; i < A.nn; ++i)
-    if ((tl >> i) & 1) base |= int64_t(1) << A.npos[i];
+  const int64_t tl = t & ((int64_t(1) << A->nn) - 1);
+  int64_t base = (t >> A->nn) << A->n;
+  for (int i = 0; i < A->nn; ++i)
+    if ((tl >> i) & 1) base |= int64_t(1) << A->npos[i];
   return base;
 }
 
 // Offset of this thread's first load/store slot: thread bit i is local bit
 // i + 1 (local bit 0 is the amplitude pair inside one 16-byte vector).
-template <int TB>
-__device__ __forceinline__ int64_t thread_offset(const PassArgs &A, int tid) {
+template <int TB, class AP>
+__device__ __forceinline__ int64_t thread_offset(AP A, int tid) {
   int64_t g = 0;
 #pragma unroll
   for (int i = 0; i < TB; ++i)
-    if ((tid >> i) & 1) g |= int64_t(1) << A.lpos[i + 1];
+    if ((tid >> i) & 1) g |= int64_t(1) << A->lpos[i + 1];
   return g;
 }
 
 // Issues the HBM loads of one tile into registers (slot it of this thread =
 // amplitude pair it * T + tid of the tile, in local-index order).  The loads
 // stay in flight until the registers are staged into LDS.
-template <int R, int TB>
-__device__ __forceinline__ void fetch_tile(const PassArgs &A, int64_t base, int64_t g_tid,
+template <int R, int TB, class AP>
+__device__ __forceinline__ void fetch_tile(AP A, int64_t base, int64_t g_tid,
                                            float4 (&pre)[1 << (R - 1)]) {
-  const int64_t total = A.B << A.n;
+  const int64_t total = A->B << A->n;
 #pragma unroll
   for (int it = 0; it < (1 << (R - 1)); ++it) {
     int64_t g_it = 0;
 #pragma unroll
     for (int i = 0; i < R - 1; ++i)
-      if ((it >> i) & 1) g_it |= int64_t(1) << A.lpos[TB + 1 + i];
+      if ((it >> i) & 1) g_it |= int64_t(1) << A->lpos[TB + 1 + i];
     const int64_t gi = base | g_tid | g_it;
-    pre[it] = gi < total ? reinterpret_cast<const float4 *>(A.src)[gi >> 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+    pre[it] = gi < total ? load_g4(A->src, gi >> 1) : make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
 
@@ -98,21 +99,21 @@ __device__ __forceinline__ void stage_tile(float2 *tile, const float4 (&pre)[1 <
 // reads a state; generated in registers when it does not), then the store.
 // `tile` (2^K float2, 16-byte aligned) and `red` ((T/64)*(K+1) floats) are
 // the caller's LDS.
-template <int R, int TB, bool MULTI>
-__device__ __forceinline__ void pass_body(const PassArgs &A, const int64_t t, const int64_t base,
+template <int R, int TB, bool MULTI, class AP>
+__device__ __forceinline__ void pass_body(AP A, const int64_t t, const int64_t base,
                                           const int64_t g_tid, float2 *tile, float *red) {
   constexpr int T = 1 << TB;
   constexpr int K = R + TB;
   constexpr int NR = 1 << R;
 
   const int tid = threadIdx.x;
-  const int n = A.n;
-  const int64_t total = A.B << n;
+  const int n = A->n;
+  const int64_t total = A->B << n;
   const int64_t block_sample = base >> n;
 
   float2 v[NR];
-  const int32_t *w = A.words;
-  for (int s = 0; s < A.nsub; ++s) {
+  kptr<int32_t> w = to_const(A->words);
+  for (int s = 0; s < A->nsub; ++s) {
     const int nseg = w[0], nphase = w[6];
     int gl = 0;
     int64_t gbase = base;
@@ -124,7 +125,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &A, const int64_t t, co
       }
     }
     const int64_t sample = MULTI ? (gbase >> n) : block_sample;
-    const bool live = sample < A.B;
+    const bool live = sample < A->B;
     const int lb = swz(gl);
     int sw[R];
 #pragma unroll
@@ -139,22 +140,22 @@ __device__ __forceinline__ void pass_body(const PassArgs &A, const int64_t t, co
       return a;
     };
 
-    if (s == 0 && !A.src) {
+    if (s == 0 && !A->src) {
       // product-state generator: (x)_pos f_pos(bit), per sample
       if (live) {
-        const float2 *E = reinterpret_cast<const float2 *>(A.efac) + sample * n * 2;
+        const kptr<int32_t> E = to_const(reinterpret_cast<const int32_t *>(A->efac)) + sample * n * 4;
         int64_t rmask = 0;
 #pragma unroll
         for (int r = 0; r < R; ++r) rmask |= int64_t(1) << w[kHdrGP + r];
         float2 pre = make_float2(1.f, 0.f);
         for (int pos = 0; pos < n; ++pos)
-          if (!((rmask >> pos) & 1)) pre = cmul(pre, E[pos * 2 + ((gbase >> pos) & 1)]);
+          if (!((rmask >> pos) & 1)) pre = cmul(pre, load_k2(E, pos * 2 + ((gbase >> pos) & 1)));
         // doubling over the register bits; both loops have constant trip
         // counts so every v[] index is a compile-time constant
         v[0] = pre;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const float2 f0 = E[w[kHdrGP + r] * 2], f1 = E[w[kHdrGP + r] * 2 + 1];
+          const float2 f0 = load_k2(E, w[kHdrGP + r] * 2), f1 = load_k2(E, w[kHdrGP + r] * 2 + 1);
 #pragma unroll
           for (int j = 0; j < (1 << r); ++j) {
             v[j | (1 << r)] = cmul(v[j], f1);
@@ -173,8 +174,8 @@ __device__ __forceinline__ void pass_body(const PassArgs &A, const int64_t t, co
     // a padding sample past B (last tile of a multi-sample pass) holds zeros;
     // it reads sample 0's coefficients so every table access stays in bounds
     const int64_t csample = live ? sample : 0;
-    const int32_t *ph = w + kHdr;
-    const int32_t *op = ph + 4 * nphase;
+    const kptr<int32_t> ph = w + kHdr;
+    kptr<int32_t> op = ph + 4 * nphase;
     for (int sg = 0; sg < nseg; ++sg) {
       const int head = op[0];
       const int umask = head & 0xff, ncnot = (head >> 8) & 0xff, psmask = (head >> 16) & 0xff;
@@ -183,7 +184,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &A, const int64_t t, co
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int64_t ui = op[1 + r] + (((psmask >> r) & 1) ? csample : 0);
-        u[r] = load_m8(A.utab + 8 * ui);
+        u[r] = load_m8(to_const(A->utab) + 8 * ui);
       }
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -201,12 +202,12 @@ __device__ __forceinline__ void pass_body(const PassArgs &A, const int64_t t, co
       op += ncnot;
     }
     if (nphase) {
-      const float2 z = phase_product(ph, nphase, A.cs, csample, gbase, false);
+      const float2 z = phase_product(ph, nphase, to_const(A->cs), csample, gbase, false);
 #pragma unroll
       for (int j = 0; j < NR; ++j) v[j] = cmul(z, v[j]);
     }
 
-    if (s == A.nsub - 1 && A.mpart) {
+    if (s == A->nsub - 1 && A->mpart) {
       // MeasureProbability partials: for every local bit, sum of |a|^2 over the
       // tile's amplitudes (of this sample) with that bit = 0; then the total.
       float p[NR];
@@ -238,10 +239,10 @@ __device__ __forceinline__ void pass_body(const PassArgs &A, const int64_t t, co
         c[e] = x;
       }
       const int spt_log = MULTI ? (K - n) : 0;
-      float *mp = A.mpart + (t << spt_log) * (K + 1);
+      const gptr<float> mp = to_global(A->mpart) + (t << spt_log) * (K + 1);
       if (seglog <= 6) {
         if ((tid & ((1 << seglog) - 1)) == 0) {
-          float *o = mp + (tid >> seglog) * (K + 1);
+          const gptr<float> o = mp + (tid >> seglog) * (K + 1);
 #pragma unroll
           for (int r = 0; r < R; ++r) o[w[kHdrP + r]] = c[r];
 #pragma unroll
@@ -279,18 +280,18 @@ __device__ __forceinline__ void pass_body(const PassArgs &A, const int64_t t, co
     w = op;
   }
 
-  if (A.store) {
+  if (A->store) {
 #pragma unroll
     for (int it = 0; it < (NR >> 1); ++it) {
       int64_t g_it = 0;
 #pragma unroll
       for (int i = 0; i < R - 1; ++i)
-        if ((it >> i) & 1) g_it |= int64_t(1) << A.lpos[TB + 1 + i];
+        if ((it >> i) & 1) g_it |= int64_t(1) << A->lpos[TB + 1 + i];
       const int64_t gi = base | g_tid | g_it;
       const int sl = swz(2 * (it * T + tid));
       float4 val = *reinterpret_cast<const float4 *>(&tile[sl & ~1]);
       if (sl & 1) val = make_float4(val.z, val.w, val.x, val.y);
-      if (gi < total) reinterpret_cast<float4 *>(A.state)[gi >> 1] = val;
+      if (gi < total) store_g4(A->state, gi >> 1, val);
     }
   }
 }
@@ -313,19 +314,19 @@ k_pass(PassArgs A, int64_t ntiles) {
   __shared__ float red[((1 << TB) / 64) * (R + TB + 1)];
   constexpr int K = R + TB;
   const int tid = threadIdx.x;
-  const int64_t g_tid = thread_offset<TB>(A, tid);
+  const int64_t g_tid = thread_offset<TB>(&A, tid);
   float4 pre[1 << (R - 1)];
   int64_t t = blockIdx.x;
-  if (A.src && t < ntiles) fetch_tile<R, TB>(A, tile_base(A, t, K, MULTI), g_tid, pre);
+  if (A.src && t < ntiles) fetch_tile<R, TB>(&A, tile_base(&A, t, K, MULTI), g_tid, pre);
   for (; t < ntiles; t += gridDim.x) {
-    const int64_t base = tile_base(A, t, K, MULTI);
+    const int64_t base = tile_base(&A, t, K, MULTI);
     if (A.src) {
       stage_tile<R, TB>(tile, pre, tid);
       __syncthreads();
       const int64_t tn = t + gridDim.x;
-      if (tn < ntiles) fetch_tile<R, TB>(A, tile_base(A, tn, K, MULTI), g_tid, pre);
+      if (tn < ntiles) fetch_tile<R, TB>(&A, tile_base(&A, tn, K, MULTI), g_tid, pre);
     }
-    pass_body<R, TB, MULTI>(A, t, base, g_tid, tile, red);
+    pass_body<R, TB, MULTI>(&A, t, base, g_tid, tile, red);
     __syncthreads();
   }
 }
@@ -340,28 +341,31 @@ k_pass_items(const PassArgs *descs, const int32_t *items, int nitems) {
   __shared__ float red[((1 << TB) / 64) * (R + TB + 1)];
   constexpr int K = R + TB;
   const int tid = threadIdx.x;
+  // descriptors and item list are read-only: uniform reads on the scalar path
+  const kptr<PassArgs> D = to_const(descs);
+  const kptr<int32_t> I = to_const(items);
   float4 pre[1 << (R - 1)];
   int i = blockIdx.x;
   if (i < nitems) {
-    const PassArgs &A = descs[items[2 * i]];
-    if (A.src) fetch_tile<R, TB>(A, tile_base(A, items[2 * i + 1], K, A.multi), thread_offset<TB>(A, tid), pre);
+    const kptr<PassArgs> A = D + I[2 * i];
+    if (A->src) fetch_tile<R, TB>(A, tile_base(A, I[2 * i + 1], K, A->multi), thread_offset<TB>(A, tid), pre);
   }
   for (; i < nitems; i += gridDim.x) {
-    const PassArgs &A = descs[items[2 * i]];
-    const int64_t t = items[2 * i + 1];
-    const int64_t base = tile_base(A, t, K, A.multi);
+    const kptr<PassArgs> A = D + I[2 * i];
+    const int64_t t = I[2 * i + 1];
+    const int64_t base = tile_base(A, t, K, A->multi);
     const int64_t g_tid = thread_offset<TB>(A, tid);
-    if (A.src) {
+    if (A->src) {
       stage_tile<R, TB>(tile, pre, tid);
       __syncthreads();
     }
     const int in = i + gridDim.x;
     if (in < nitems) {
-      const PassArgs &An = descs[items[2 * in]];
-      if (An.src)
-        fetch_tile<R, TB>(An, tile_base(An, items[2 * in + 1], K, An.multi), thread_offset<TB>(An, tid), pre);
+      const kptr<PassArgs> An = D + I[2 * in];
+      if (An->src)
+        fetch_tile<R, TB>(An, tile_base(An, I[2 * in + 1], K, An->multi), thread_offset<TB>(An, tid), pre);
     }
-    if (A.multi)
+    if (A->multi)
       pass_body<R, TB, true>(A, t, base, g_tid, tile, red);
     else
       pass_body<R, TB, false>(A, t, base, g_tid, tile, red);
@@ -807,6 +811,10 @@ int qs_items_pass(const void *descs, const int32_t *items, int nitems, int R, in
   }
   if (R == 5 && TB == 6) {
     hipLaunchKernelGGL((k_pass_items<5, 6>), dim3(g), dim3(64), 0, st, d, items, nitems);
+    return launch_status();
+  }
+  if (R == 5 && TB == 8) {
+    hipLaunchKernelGGL((k_pass_items<5, 8>), dim3(g), dim3(256), 0, st, d, items, nitems);
     return launch_status();
   }
   return QS_EUNSUPPORTED;

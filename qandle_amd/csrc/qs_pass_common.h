@@ -27,6 +27,26 @@ __host__ __device__ __forceinline__ int swz(int l) {
   return l ^ (f & 31);
 }
 
+// Address-space views of the buffers a descriptor points to.  A pointer read
+// from memory (a work item's descriptor) is generic: its loads and stores are
+// flat instructions, which count against the LDS counter too, so every LDS
+// wait would also wait for the next tile's HBM loads.  The state buffers are
+// global memory; the programs and tables are read-only while a kernel runs,
+// and reading them through the constant address space puts uniform reads on
+// the scalar path (and the branches on their words on scalar branches).
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+template <class T>
+using kptr = const __attribute__((address_space(4))) T *;
+template <class T>
+__device__ __forceinline__ gptr<T> to_global(T *p) {
+  return (gptr<T>)p;
+}
+template <class T>
+__device__ __forceinline__ kptr<T> to_const(const T *p) {
+  return (kptr<T>)p;
+}
+
 // ---------------------------------------------------------------------------
 // Per-bit operations.  A 2x2 complex matrix is 8 floats
 // (u00.re, u00.im, u01.re, u01.im, u10.re, u10.im, u11.re, u11.im), column
@@ -42,7 +62,8 @@ struct M8 {
   float x[8];
 };
 
-__device__ __forceinline__ M8 load_m8(const int32_t *p) {
+template <class P>
+__device__ __forceinline__ M8 load_m8(P p) {
   M8 m;
 #pragma unroll
   for (int i = 0; i < 8; ++i) m.x[i] = __int_as_float(p[i]);
@@ -83,6 +104,21 @@ __device__ __forceinline__ M8 dagger(const M8 &u) {
 // U times its pair.  The matrix units run at twice the FP32 rate of the
 // vector ALU and leave it free for the rest of the pass.
 typedef float qs_f4 __attribute__((ext_vector_type(4)));
+
+// 16-byte state moves and 8-byte table reads through the address-space views
+// (the HIP vector classes have no operators for address-space-qualified
+// objects; clang's native vectors and scalars do)
+__device__ __forceinline__ float4 load_g4(const float *p, int64_t i) {
+  const qs_f4 v = to_global(reinterpret_cast<const qs_f4 *>(p))[i];
+  return make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void store_g4(float *p, int64_t i, const float4 &v) {
+  qs_f4 x = {v.x, v.y, v.z, v.w};
+  to_global(reinterpret_cast<qs_f4 *>(p))[i] = x;
+}
+__device__ __forceinline__ float2 load_k2(kptr<int32_t> p, int64_t i) {
+  return make_float2(__int_as_float(p[2 * i]), __int_as_float(p[2 * i + 1]));
+}
 
 struct MU {
   float a[4];  // A_k of this lane, k = 0..3
@@ -184,8 +220,9 @@ __device__ __forceinline__ M8 cross_density_at(const float2 (&p)[1 << R], const 
 // Per-thread phase of a sub-pass's RZ gates on non-register bits:
 // prod_k e^{-/+ i phi_k/2} by the thread's value of bit pos_k.  cs holds the
 // (cos, sin) float pairs as int32 bits.
-__device__ __forceinline__ float2 phase_product(const int32_t *ph, int nphase, const int32_t *cs, int64_t csample,
-                                                int64_t gbase, bool inverse) {
+template <class PH, class CS>
+__device__ __forceinline__ float2 phase_product(PH ph, int nphase, CS cs, int64_t csample, int64_t gbase,
+                                                bool inverse) {
   float pr = 1.f, pi = 0.f;
   for (int k = 0; k < nphase; ++k) {
     const int fl = ph[4 * k + 2];
