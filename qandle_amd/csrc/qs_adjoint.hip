@@ -132,9 +132,23 @@ __device__ __forceinline__ void adjoint_tile(AP A, const int64_t t, float2 *tp, 
       vl[j] = tl[addr(j)];
     }
 
-    auto reduce_put = [&](float g, int row) {
+    // A segment's sums go to its reduction slot u.  Whole-wave segments sum
+    // four rows at a time with DPP and lane swaps (wave_sum4; lanes 0..3 store
+    // them); narrower segments (several samples per wave) reduce by shuffles.
+    const bool wave_slots = wlog == 6;
+    const gptr<float> gp = to_global(A->gpart);
+    auto put1 = [&](float g, int row) {
       for (int off = 1; off < (1 << wlog); off <<= 1) g += __shfl_xor(g, off, 64);
-      if (leader && row >= 0) to_global(A->gpart)[(int64_t)row * A->NS + u] = g;
+      if (leader && row >= 0) gp[(int64_t)row * A->NS + u] = g;
+    };
+    auto put4 = [&](const float (&tv)[4], const int (&rw)[4]) {
+      const float g = wave_sum4(tv[0], tv[1], tv[2], tv[3]);
+      const int ln = tid & 63;
+      if (ln < 4) {
+        const int v = wave_sum4_index(ln);
+        const int row = v == 0 ? rw[0] : v == 1 ? rw[1] : v == 2 ? rw[2] : rw[3];
+        if (row >= 0) gp[(int64_t)row * A->NS + u] = g;
+      }
     };
     const kptr<int32_t> ph = w + kHdr;
     kptr<int32_t> op = ph + 4 * nphase;
@@ -144,7 +158,22 @@ __device__ __forceinline__ void adjoint_tile(AP A, const int64_t t, float2 *tp, 
       float S = 0.f;
 #pragma unroll
       for (int j = 0; j < NR; ++j) S += vl[j].x * vp[j].y - vl[j].y * vp[j].x;
-      for (int k = 0; k < nphase; ++k) reduce_put(((gbase >> ph[4 * k]) & 1) ? -S : S, ph[4 * k + 3]);
+      if (wave_slots) {
+        for (int k0 = 0; k0 < nphase; k0 += 4) {
+          float tv[4];
+          int rw[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const bool ok = k0 + c < nphase;
+            const int k = ok ? k0 + c : k0;
+            tv[c] = ok ? (((gbase >> ph[4 * k]) & 1) ? -S : S) : 0.f;
+            rw[c] = ok ? ph[4 * k + 3] : -1;
+          }
+          put4(tv, rw);
+        }
+      } else {
+        for (int k = 0; k < nphase; ++k) put1(((gbase >> ph[4 * k]) & 1) ? -S : S, ph[4 * k + 3]);
+      }
       const float2 z = phase_product(ph, nphase, to_const(A->cs), csample, gbase, true);
 #pragma unroll
       for (int j = 0; j < NR; ++j) {
@@ -169,9 +198,25 @@ __device__ __forceinline__ void adjoint_tile(AP A, const int64_t t, float2 *tp, 
           const bool ps = (psmask >> r) & 1;
           const int64_t var = ps ? csample : 0;
           const int64_t span = ps ? A->B : 1;
-          const M8 m = cross_density_at<R>(vp, vl, r);
-          for (int k = 0; k < nconst; ++k)
-            reduce_put(im_trace(load_m8(to_const(A->qtab) + 8 * (op[2] + k * span + var)), m), op[3 + k]);
+          const H4 m = herm_density_at<R>(vp, vl, r);
+          const kptr<int32_t> qt = to_const(A->qtab);
+          if (wave_slots) {
+            for (int k0 = 0; k0 < nconst; k0 += 4) {
+              float tv[4];
+              int rw[4];
+#pragma unroll
+              for (int c = 0; c < 4; ++c) {
+                const bool ok = k0 + c < nconst;
+                const int k = ok ? k0 + c : k0;
+                const float tr = herm_trace(qt + 8 * (op[2] + k * span + var), m);
+                tv[c] = ok ? tr : 0.f;
+                rw[c] = ok ? op[3 + k] : -1;
+              }
+              put4(tv, rw);
+            }
+          } else {
+            for (int k = 0; k < nconst; ++k) put1(herm_trace(qt + 8 * (op[2] + k * span + var), m), op[3 + k]);
+          }
           if (!((probemask >> r) & 1)) {
             const M8 uu = load_m8(to_const(A->utab) + 8 * (op[0] + var));
             if (MULTI && ps) {  // may differ inside a 4-lane matrix block

@@ -162,24 +162,79 @@ __device__ __forceinline__ void apply_mfma(float2 (&v)[NR], const MU &m) {
   }
 }
 
-// Cross density of bit BIT over the thread's registers: M_ab = sum psi_a conj(lam_b),
-// 8 floats (m00.re, m00.im, m01.re, m01.im, m10.re, m10.im, m11.re, m11.im).
+// The cross density M_ab = sum psi_a conj(lam_b) of bit BIT over the thread's
+// registers enters a gradient only through Im Tr(Q M), and every gradient
+// generator Q (a Pauli transported through the rest of its fused unitary,
+// V P V^dagger) is Hermitian: Q00 and Q11 are real and Q10 = conj(Q01), so
+// Im Tr(Q M) = Q00 Im M00 + Q11 Im M11 + Re Q01 (Im M01 + Im M10)
+//            + Im Q10 (Re M01 - Re M10).
+// herm_density accumulates those four combinations (four fused multiply-add
+// chains instead of the eight entries of M).
+struct H4 {
+  float h[4];
+};
+
 template <int BIT, int NR>
-__device__ __forceinline__ M8 cross_density(const float2 (&p)[NR], const float2 (&l)[NR]) {
-  float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f, m6 = 0.f, m7 = 0.f;
+__device__ __forceinline__ H4 herm_density(const float2 (&p)[NR], const float2 (&l)[NR]) {
+  float h0 = 0.f, h1 = 0.f, h2 = 0.f, h3 = 0.f;
 #pragma unroll
   for (int j = 0; j < NR; ++j) {
     if (!(j & (1 << BIT))) {
       const float2 p0 = p[j], p1 = p[j | (1 << BIT)], l0 = l[j], l1 = l[j | (1 << BIT)];
-      // psi_a * conj(lam_b) = (pa.x lb.x + pa.y lb.y) + i (pa.y lb.x - pa.x lb.y)
-      m0 += p0.x * l0.x + p0.y * l0.y;  m1 += p0.y * l0.x - p0.x * l0.y;
-      m2 += p0.x * l1.x + p0.y * l1.y;  m3 += p0.y * l1.x - p0.x * l1.y;
-      m4 += p1.x * l0.x + p1.y * l0.y;  m5 += p1.y * l0.x - p1.x * l0.y;
-      m6 += p1.x * l1.x + p1.y * l1.y;  m7 += p1.y * l1.x - p1.x * l1.y;
+      h0 = fmaf(p0.y, l0.x, fmaf(-p0.x, l0.y, h0));
+      h1 = fmaf(p1.y, l1.x, fmaf(-p1.x, l1.y, h1));
+      h2 = fmaf(p0.y, l1.x, fmaf(-p0.x, l1.y, fmaf(p1.y, l0.x, fmaf(-p1.x, l0.y, h2))));
+      h3 = fmaf(p0.x, l1.x, fmaf(p0.y, l1.y, fmaf(-p1.x, l0.x, fmaf(-p1.y, l0.y, h3))));
     }
   }
-  return M8{{m0, m1, m2, m3, m4, m5, m6, m7}};
+  return H4{{h0, h1, h2, h3}};
 }
+
+// Im Tr(Q M) from herm_density for a Hermitian generator stored as 8 floats
+// (q00.re, q00.im, q01.re, q01.im, q10.re, q10.im, q11.re, q11.im), int bits.
+template <class P>
+__device__ __forceinline__ float herm_trace(P q, const H4 &m) {
+  return (__int_as_float(q[0]) * m.h[0] + __int_as_float(q[6]) * m.h[1]) +
+         (__int_as_float(q[4]) * m.h[2] + __int_as_float(q[5]) * m.h[3]);
+}
+
+// Wave reductions without LDS traffic.  dpp_add<CTRL> adds the value of the
+// lane a DPP control selects: quad_perm [1,0,3,2] (0xB1) and [2,3,0,1] (0x4E)
+// exchange within quads, row_ror 4 (0x124) and 8 (0x128) rotate a 16-lane
+// row.  The gfx950 row and half swaps (permlane16_swap, permlane32_swap) with
+// the same value as both operands return the value's two halves, whose sum
+// is s[l] + s[l ^ 16] (s[l] + s[l ^ 32]) in every lane.
+template <int CTRL>
+__device__ __forceinline__ float dpp_add(float s) {
+  return s + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(s), CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float swap16_add(float s) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(s), __float_as_uint(s), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float swap32_add(float s) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(s), __float_as_uint(s), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+// Four values summed over a 64-lane wave together: two exchange rounds (lane
+// l ^ 1, then l ^ 2) leave each lane one value's quad sum, value index
+// 2 (l & 1) + ((l >> 1) & 1); the row rotations add a row's lanes of the same
+// index and the two swaps the four rows.  Lanes 0, 2, 1, 3 (and every lane of
+// the same index) end with the wave sums of values 0, 1, 2, 3.
+__device__ __forceinline__ float wave_sum4(float t0, float t1, float t2, float t3) {
+  const int l = threadIdx.x;
+  const bool b0 = l & 1, b1 = l & 2;
+  const float k0 = b0 ? t2 : t0, k1 = b0 ? t3 : t1, s0 = b0 ? t0 : t2, s1 = b0 ? t1 : t3;
+  const float u0 = k0 + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(s0), 0xB1, 0xF, 0xF, false));
+  const float u1 = k1 + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(s1), 0xB1, 0xF, 0xF, false));
+  const float k = b1 ? u1 : u0, s = b1 ? u0 : u1;
+  float w = k + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(s), 0x4E, 0xF, 0xF, false));
+  w = dpp_add<0x128>(dpp_add<0x124>(w));
+  return swap32_add(swap16_add(w));
+}
+// the value index wave_sum4 leaves in lane l
+__device__ __forceinline__ int wave_sum4_index(int l) { return 2 * (l & 1) + ((l >> 1) & 1); }
 
 // run-time bit -> compile-time template (the switch folds away once the
 // caller's loop over register bits is unrolled)
@@ -206,15 +261,15 @@ __device__ __forceinline__ void apply_mfma_at(float2 (&v)[1 << R], int r, const 
   }
 }
 template <int R>
-__device__ __forceinline__ M8 cross_density_at(const float2 (&p)[1 << R], const float2 (&l)[1 << R], int r) {
+__device__ __forceinline__ H4 herm_density_at(const float2 (&p)[1 << R], const float2 (&l)[1 << R], int r) {
   switch (r) {
-    case 1: if constexpr (R > 1) return cross_density<1>(p, l); break;
-    case 2: if constexpr (R > 2) return cross_density<2>(p, l); break;
-    case 3: if constexpr (R > 3) return cross_density<3>(p, l); break;
-    case 4: if constexpr (R > 4) return cross_density<4>(p, l); break;
+    case 1: if constexpr (R > 1) return herm_density<1>(p, l); break;
+    case 2: if constexpr (R > 2) return herm_density<2>(p, l); break;
+    case 3: if constexpr (R > 3) return herm_density<3>(p, l); break;
+    case 4: if constexpr (R > 4) return herm_density<4>(p, l); break;
     default: break;
   }
-  return cross_density<0>(p, l);
+  return herm_density<0>(p, l);
 }
 
 // Per-thread phase of a sub-pass's RZ gates on non-register bits:
@@ -236,13 +291,6 @@ __device__ __forceinline__ float2 phase_product(PH ph, int nphase, CS cs, int64_
     pi = ni;
   }
   return make_float2(pr, pi);
-}
-
-// Im Tr(Q M) = Im sum_ab Q_ab M_ba
-__device__ __forceinline__ float im_trace(const M8 &q, const M8 &m) {
-  // Q00 M00 + Q01 M10 + Q10 M01 + Q11 M11, imaginary parts
-  return (q.x[0] * m.x[1] + q.x[1] * m.x[0]) + (q.x[2] * m.x[5] + q.x[3] * m.x[4]) +
-         (q.x[4] * m.x[3] + q.x[5] * m.x[2]) + (q.x[6] * m.x[7] + q.x[7] * m.x[6]);
 }
 
 template <int CB, int TBIT, int NR>
