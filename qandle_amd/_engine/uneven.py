@@ -378,7 +378,7 @@ class UnevenBatch:
         for items, cnt in zip(self.bw_levels, self.bw_level_items):
             L.check(lib.qs_items_adjoint(L.ptr(self.bw_descs), L.ptr(items), cnt, self.R, self.TB, self.grid, st),
                     "qs_items_adjoint")
-        L.check(lib.qs_items_adjoint_finish(L.ptr(self.f_descs), len(self.circs), 4, st),
+        L.check(lib.qs_items_adjoint_finish(L.ptr(self.f_descs), len(self.circs), 8, st),
                 "qs_items_adjoint_finish")
         self.grad_phi.zero_()
         if self.np_rows:

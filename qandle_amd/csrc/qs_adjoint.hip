@@ -141,8 +141,7 @@ __device__ __forceinline__ void adjoint_tile(AP A, const int64_t t, float2 *tp, 
       for (int off = 1; off < (1 << wlog); off <<= 1) g += __shfl_xor(g, off, 64);
       if (leader && row >= 0) gp[(int64_t)row * A->NS + u] = g;
     };
-    auto put4 = [&](const float (&tv)[4], const int (&rw)[4]) {
-      const float g = wave_sum4(tv[0], tv[1], tv[2], tv[3]);
+    auto store4 = [&](float g, const int (&rw)[4]) {  // g: a wave_sum4 result
       const int ln = tid & 63;
       if (ln < 4) {
         const int v = wave_sum4_index(ln);
@@ -169,7 +168,7 @@ __device__ __forceinline__ void adjoint_tile(AP A, const int64_t t, float2 *tp, 
             tv[c] = ok ? (((gbase >> ph[4 * k]) & 1) ? -S : S) : 0.f;
             rw[c] = ok ? ph[4 * k + 3] : -1;
           }
-          put4(tv, rw);
+          store4(wave_sum4(tv[0], tv[1], tv[2], tv[3]), rw);
         }
       } else {
         for (int k = 0; k < nphase; ++k) put1(((gbase >> ph[4 * k]) & 1) ? -S : S, ph[4 * k + 3]);
@@ -196,36 +195,57 @@ __device__ __forceinline__ void adjoint_tile(AP A, const int64_t t, float2 *tp, 
         if ((umask >> r) & 1) {
           const int nconst = op[1];
           const bool ps = (psmask >> r) & 1;
+          const bool probe = (probemask >> r) & 1;
           const int64_t var = ps ? csample : 0;
           const int64_t span = ps ? A->B : 1;
-          const H4 m = herm_density_at<R>(vp, vl, r);
           const kptr<int32_t> qt = to_const(A->qtab);
-          if (wave_slots) {
-            for (int k0 = 0; k0 < nconst; k0 += 4) {
-              float tv[4];
-              int rw[4];
+          const M8 uu = load_m8(to_const(A->utab) + 8 * (op[0] + var));  // a probe's entry is the identity
+          const H4 m = herm_density_at<R>(vp, vl, r);  // at the gate's output
+          // gradient rows k0, k0 + 1, ... (four at a time): traces and the wave sum
+          auto sum4 = [&](int k0, int (&rw)[4]) {
+            float tv[4];
 #pragma unroll
-              for (int c = 0; c < 4; ++c) {
-                const bool ok = k0 + c < nconst;
-                const int k = ok ? k0 + c : k0;
-                const float tr = herm_trace(qt + 8 * (op[2] + k * span + var), m);
-                tv[c] = ok ? tr : 0.f;
-                rw[c] = ok ? op[3 + k] : -1;
-              }
-              put4(tv, rw);
+            for (int c = 0; c < 4; ++c) {
+              const bool ok = k0 + c < nconst;
+              const int k = ok ? k0 + c : k0;
+              const float tr = herm_trace(qt + 8 * (op[2] + k * span + var), m);
+              tv[c] = ok ? tr : 0.f;
+              rw[c] = ok ? op[3 + k] : -1;
+            }
+            return wave_sum4(tv[0], tv[1], tv[2], tv[3]);
+          };
+          if (wave_slots && !probe && !(MULTI && ps)) {
+            // the first four rows' traces and wave sum sit in one block with the
+            // U^dagger matrix-core chains, so the scheduler can interleave them
+            int rw[4];
+            const float g = sum4(0, rw);
+            const MU ud = mfma_coeffs<true>(uu);
+            apply_mfma_at<R>(vp, r, ud);
+            apply_mfma_at<R>(vl, r, ud);
+            store4(g, rw);
+            for (int k0 = 4; k0 < nconst; k0 += 4) {
+              const float g2 = sum4(k0, rw);
+              store4(g2, rw);
             }
           } else {
-            for (int k = 0; k < nconst; ++k) put1(herm_trace(qt + 8 * (op[2] + k * span + var), m), op[3 + k]);
-          }
-          if (!((probemask >> r) & 1)) {
-            const M8 uu = load_m8(to_const(A->utab) + 8 * (op[0] + var));
-            if (MULTI && ps) {  // may differ inside a 4-lane matrix block
-              apply_valu_at<R>(vp, r, dagger(uu));
-              apply_valu_at<R>(vl, r, dagger(uu));
+            if (wave_slots) {
+              for (int k0 = 0; k0 < nconst; k0 += 4) {
+                int rw[4];
+                const float g = sum4(k0, rw);
+                store4(g, rw);
+              }
             } else {
-              const MU ud = mfma_coeffs<true>(uu);
-              apply_mfma_at<R>(vp, r, ud);
-              apply_mfma_at<R>(vl, r, ud);
+              for (int k = 0; k < nconst; ++k) put1(herm_trace(qt + 8 * (op[2] + k * span + var), m), op[3 + k]);
+            }
+            if (!probe) {
+              if (MULTI && ps) {  // may differ inside a 4-lane matrix block
+                apply_valu_at<R>(vp, r, dagger(uu));
+                apply_valu_at<R>(vl, r, dagger(uu));
+              } else {
+                const MU ud = mfma_coeffs<true>(uu);
+                apply_mfma_at<R>(vp, r, ud);
+                apply_mfma_at<R>(vl, r, ud);
+              }
             }
           }
           op += 3 + nconst;
@@ -357,11 +377,67 @@ __global__ void __launch_bounds__(256) k_adjoint_finish(AdjFinishArgs F) {
   adjoint_finish_row(F, blockIdx.x, red);
 }
 
-// one block per (circuit, row block)
+// One wave per gradient row (the rows of a batch of circuits are many and
+// short: a circuit of batch 1 has at most a few hundred slots per row).  The
+// same sums as adjoint_finish_row, in double, with no block barrier between
+// rows, so every wave of the launch keeps its own rows' loads in flight.
+__device__ __forceinline__ void adjoint_finish_row_wave(kptr<AdjFinishArgs> F, int64_t row, int lane) {
+  const gptr<const float> g = to_global(F->gpart) + row * F->NS;
+  const gptr<float> out = to_global(F->out) + row * F->B;
+  const int64_t B = F->B, sps = F->sps;
+  double all = 0.0;
+  if (sps <= 64) {
+    const int64_t total = B * sps;
+    const int seg = (int)sps;
+    for (int64_t c0 = 0; c0 < total; c0 += 512) {
+      double acc[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int64_t i = c0 + u * 64 + lane;
+        acc[u] = i < total ? (double)g[i] : 0.0;
+      }
+      for (int off = 1; off < seg; off <<= 1) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[u] += __shfl_xor(acc[u], off, 64);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int64_t i = c0 + u * 64 + lane;
+        if ((lane & (seg - 1)) == 0 && i < total) {
+          out[i / seg] = (float)(0.5 * acc[u]);
+          all += 0.5 * acc[u];
+        }
+      }
+    }
+  } else {
+    for (int64_t b = 0; b < B; ++b) {
+      const gptr<const float> gb = g + b * sps;
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int64_t k = lane; k < sps; k += 256) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] += k + 64 * u < sps ? (double)gb[k + 64 * u] : 0.0;
+      }
+      double a = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) a += __shfl_xor(a, off, 64);
+      if (lane == 0) {
+        out[b] = (float)(0.5 * a);
+        all += 0.5 * a;
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) all += __shfl_xor(all, off, 64);
+  if (lane == 0) to_global(F->tot)[row] = (float)all;
+}
+
+// Block (circuit, row block): wave w of row block y takes rows 4 y + w, then
+// every 4 gridDim.y rows after it.
 __global__ void __launch_bounds__(256) k_adjoint_finish_items(const AdjFinishArgs *descs) {
-  __shared__ double red[4];
-  const AdjFinishArgs &F = descs[blockIdx.x];
-  for (int64_t row = blockIdx.y; row < F.rows; row += gridDim.y) adjoint_finish_row(F, row, red);
+  const kptr<AdjFinishArgs> F = to_const(descs) + blockIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  for (int64_t row = 4 * (int64_t)blockIdx.y + wave; row < F->rows; row += 4 * (int64_t)gridDim.y)
+    adjoint_finish_row_wave(F, row, lane);
 }
 
 }  // namespace qs
