@@ -24,7 +24,7 @@ beside the same circuit on the GPU).  ``cpu_baseline``: the structured
 complex64 restatement of qandle's gate semantics on the host's CPU share.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick] [--no-cpu]
-    python bench.py --profile fused|adjoint|uneven|copy   (short runs for rocprofv3)
+    python bench.py --profile fused|adjoint|uneven|uneven_train|copy   (short runs for rocprofv3)
     python bench.py --summarize RESULTS_DB [--last N]    (steady-state kernel table)
 """
 from __future__ import annotations
@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--quick", action="store_true", help="headline only (no extra workloads, no CPU baseline)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile", choices=["fused", "adjoint", "uneven", "copy"],
+    ap.add_argument("--profile", choices=["fused", "adjoint", "uneven", "uneven_train", "copy"],
                     help="run only the named workload a few times (for rocprofv3 counter and trace runs)")
     ap.add_argument("--summarize", metavar="DB", help="print the kernel table of a rocprofv3 results database")
     ap.add_argument("--last", type=int, default=0,
@@ -123,18 +123,25 @@ def profile_workload(mode, reps=4):
         sync()
         print(f"copy: {a.numel() * 8} bytes read and written per launch", flush=True)
         return
-    if mode == "uneven":
+    if mode in ("uneven", "uneven_train"):
         from qandle_amd._engine import fused
         from qandle_amd._engine.uneven import UnevenBatch
 
         ws, lw = uneven_workloads()
         ub = UnevenBatch.from_workloads(ws, dev, R=4, TB=8, cost=fused.UNEVEN_COST)
         ub.set_loss_weights(lw)
-        settle(ub.run_forward, sync)
-        for _ in range(reps):
+
+        def run():
             ub.run_forward()
+            if mode == "uneven_train":
+                ub.run_backward()
+
+        settle(run, sync)
+        for _ in range(reps):
+            run()
         sync()
-        print(f"uneven: {reps} forwards of 2048 circuits", flush=True)
+        what = "forwards" if mode == "uneven" else "forward + backward steps"
+        print(f"uneven: {reps} {what} of 2048 circuits", flush=True)
         return
     w = headline_workload() if mode == "fused" else adjoint_workload()
     circ = qandle.Circuit(w.layers, num_qubits=w.n).to(dev)
