@@ -189,9 +189,14 @@ def plan(n: int, gates: typing.List[Gate], R: int, TB: int, cm: CostModel = None
         if best is None or score < best[0]:
             best = (score, pl)
     if cm.lookahead and gates:
-        pl = _plan_lookahead(n, gates, R, TB, cm)
-        if _score(pl) < best[0]:
-            best = (_score(pl), pl)
+        # also with one always-local position fewer (runs of 2^3 amplitudes, 64 bytes): the
+        # windows then reach one position further, which can save passes (24-qubit ladder: 7
+        # instead of 9 at K = 13, 3.03 vs 3.16-3.20 ms per forward; at 20 qubits 5 passes of 29
+        # sub-passes score the same as 6 of 25 and measured the same, 2.36 vs 2.38 ms)
+        for low in sorted({cm.min_low, max(cm.min_low - 1, 3)}, reverse=True):
+            pl = _plan_lookahead(n, gates, R, TB, dataclasses.replace(cm, min_low=low))
+            if _score(pl) < best[0]:
+                best = (_score(pl), pl)
     return best[1]
 
 

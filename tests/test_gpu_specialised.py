@@ -32,7 +32,7 @@ def both(fn):
     return a, b
 
 
-@pytest.mark.parametrize("n,B,depth", [(8, 32, 3), (13, 3, 2), (16, 2, 2), (20, 2, 10)])
+@pytest.mark.parametrize("n,B,depth", [(8, 32, 3), (13, 3, 2), (16, 2, 2), (20, 2, 10), (21, 2, 10)])
 def test_specialised_ladder_vs_fp64(device, specialise, n, B, depth):
     w = workloads.rxryrz_ladder(n=n, B=B, depth=depth, seed=n + 1)
     circ = qandle.Circuit(w.layers, num_qubits=n).to(device)

@@ -223,3 +223,24 @@ def test_lookahead_plan_matches_oracle_and_scores_no_worse(mode):
     np.testing.assert_allclose(probs, fp64.measure_prob(ref, n), atol=1e-12)
     score = lambda p: 4 * len(p.passes) + sum(len(s.subs) for s in p.passes)  # noqa: E731
     assert score(PL.plan(n, gates, R, TB, look)) <= score(PL.plan(n, gates, R, TB, greedy))
+
+
+def test_lookahead_tries_one_fewer_always_local_position():
+    """The lookahead planner also plans with positions 0-2 always local (tile
+    runs of 2^3 amplitudes) and keeps that plan when it scores strictly lower:
+    the 24-qubit ladder then runs in 7 passes instead of 9, while the
+    20-qubit one keeps its 6 passes over positions 0-3."""
+    import qandle_amd as qandle
+    from qandle_amd import workloads
+    from qandle_amd._engine import fused
+
+    def plan_of(n, B):
+        w = workloads.rxryrz_ladder(n=n, B=B, depth=10, seed=1)
+        prog = fused.lower_circuit(qandle.Circuit(w.layers, num_qubits=n).circuit)
+        p = fused.leading_rotations(prog)[0]
+        return fused.get_plan(p, 5, 8, None, fused.folds(p, True))
+
+    p24, p20 = plan_of(24, 4), plan_of(20, 64)
+    assert len(p24.passes) == 7 and any(3 not in q.L for q in p24.passes)
+    assert all({0, 1, 2} <= set(q.L) for q in p24.passes)
+    assert len(p20.passes) == 6 and all({0, 1, 2, 3} <= set(q.L) for q in p20.passes)
