@@ -212,9 +212,9 @@ def gradient_scatter(prog, rows, device):
             elif sl.kind == "named":
                 named.setdefault(sl.name, []).append(rows[gi])
         hit = {
-            "param": (torch.tensor([p for p, _ in param_rows], device=device),
-                      torch.tensor([r for _, r in param_rows], device=device)) if param_rows else None,
-            "named": {nm: torch.tensor(rr, device=device) for nm, rr in named.items()},
+            "param": (L.upload([p for p, _ in param_rows], torch.long, device),
+                      L.upload([r for _, r in param_rows], torch.long, device)) if param_rows else None,
+            "named": {nm: L.upload(rr, torch.long, device) for nm, rr in named.items()},
         }
         prog.words_cache[key] = hit
     return hit
@@ -242,7 +242,7 @@ def specialised_adjoint(prog, pl, ws, B, embed, seeded, fold=False):
     hit = prog.words_cache.get(key)
     if hit is None:
         masks = fused.readout_fold(prog)[1] if fold else None
-        hit = jit.AdjointKernels(ws.passes, prog.n, pl.R, pl.TB, seeded, readout=masks, **opts)
+        hit = jit.AdjointKernels(ws.host_passes(), prog.n, pl.R, pl.TB, seeded, readout=masks, **opts)
         prog.words_cache[key] = hit
     return hit
 

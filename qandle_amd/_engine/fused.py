@@ -463,7 +463,7 @@ def _gather_thetas(thetas, cache=None) -> torch.Tensor:
         pos = [0] * len(order)
         for j, i in enumerate(order):
             pos[i] = j
-        idx = torch.tensor(pos, dtype=torch.long, device=flat.device)
+        idx = L.upload(pos, torch.long, flat.device)
         if cache is not None:
             cache[key] = idx
     return flat.index_select(0, idx)
@@ -505,7 +505,7 @@ def param_angles(prog: Program, device) -> torch.Tensor:
         key = ("values", str(device))
         vt = prog.words_cache.get(key)
         if vt is None:
-            vt = torch.tensor([params[k].value for k in values], dtype=torch.float32).to(device)
+            vt = L.upload([params[k].value for k in values], torch.float32, device)
             prog.words_cache[key] = vt
         pieces.append(vt)
         order.append(values)
@@ -518,14 +518,14 @@ def param_angles(prog: Program, device) -> torch.Tensor:
             pos = [0] * len(flat)
             for i, k in enumerate(flat):
                 pos[k] = i
-            inv = torch.tensor(pos, dtype=torch.long, device=device)
+            inv = L.upload(pos, torch.long, device)
             prog.words_cache[key] = inv
         phi = phi.index_select(0, inv)
     if any(sl.sign != 1.0 for sl in params):
         key = ("signs", str(device))
         sg = prog.words_cache.get(key)
         if sg is None:
-            sg = torch.tensor([sl.sign for sl in params], dtype=torch.float32, device=device)
+            sg = L.upload([sl.sign for sl in params], torch.float32, device)
             prog.words_cache[key] = sg
         phi = phi * sg
     return phi
@@ -570,12 +570,17 @@ class WordSet:
     nrec: int
     nu: int                       # U entries
     nq: int                       # gradient-generator entries
+    host: list = None             # the same passes with their words as host numpy arrays
+
+    def host_passes(self):
+        """The passes with host words (what the kernel compilers read; no device copy back)."""
+        return self.host if self.host is not None else self.passes
 
 
-def upload_tables(passes, tables: PL.TableBuilder, device) -> WordSet:
-    rec = torch.from_numpy(tables.recipes()).to(device)
-    off = torch.from_numpy(np.asarray(tables.offsets or [0], dtype=np.int32)).to(device)
-    return WordSet(passes, rec, off, tables.count, tables.nu, tables.nq)
+def upload_tables(passes, tables: PL.TableBuilder, device, host=None) -> WordSet:
+    rec = L.upload(tables.recipes(), torch.int32, device)
+    off = L.upload(np.asarray(tables.offsets or [0], dtype=np.int32), torch.int32, device)
+    return WordSet(passes, rec, off, tables.count, tables.nu, tables.nq, host)
 
 
 def compose_tables(ws: WordSet, cs: torch.Tensor, generators: bool = True):
@@ -603,11 +608,12 @@ def pass_words(prog: Program, pl: PL.Plan, B: int, device) -> WordSet:
         return cached
     index, per_sample, _, _ = coefficient_layout(prog, B)
     tables = PL.TableBuilder(B)
-    out = []
+    out, host = [], []
     for p in pl.passes:
         w, nsub, lpos = _encode(p, prog.gates, pl, index.__getitem__, per_sample.__getitem__, tables)
-        out.append((torch.from_numpy(w).to(device), nsub, lpos))
-    ws = upload_tables(out, tables, device)
+        out.append((L.upload(w, torch.int32, device), nsub, lpos))
+        host.append((w, nsub, lpos))
+    ws = upload_tables(out, tables, device, host)
     prog.words_cache[key] = ws
     return ws
 
@@ -629,11 +635,12 @@ def adjoint_words(prog: Program, pl: PL.Plan, B: int, device, embed: bool, cm: P
     index, per_sample, _, _ = coefficient_layout(prog, B)
     rows = rotation_rows(prog)
     tables = PL.TableBuilder(B)
-    out = []
+    out, host = [], []
     for p in reversed(pl.passes):
         w, nsub, lpos = _encode(p, prog.gates, pl, index.__getitem__, per_sample.__getitem__, tables,
                                 rows.__getitem__)
-        out.append((torch.from_numpy(w).to(device), nsub, lpos))
+        out.append((L.upload(w, torch.int32, device), nsub, lpos))
+        host.append((w, nsub, lpos))
     nmain = len(out)
     if embed:
         gates = embed_grad_gates(prog)
@@ -642,8 +649,9 @@ def adjoint_words(prog: Program, pl: PL.Plan, B: int, device, embed: bool, cm: P
         for p in gplan.passes:
             w, nsub, lpos = _encode(p, gates, gplan, lambda s_: 0, lambda s_: False, tables,
                                     lambda gi: row0 + gi)
-            out.append((torch.from_numpy(w).to(device), nsub, lpos))
-    hit = (upload_tables(out, tables, device), nmain)
+            out.append((L.upload(w, torch.int32, device), nsub, lpos))
+            host.append((w, nsub, lpos))
+    hit = (upload_tables(out, tables, device, host), nmain)
     prog.words_cache[key] = hit
     return hit
 
@@ -713,12 +721,10 @@ def leading_rotations(prog: Program):
     return hit
 
 
-def absorb_factors(prog: Program, per_pos, efac: torch.Tensor, cs: torch.Tensor, B: int) -> torch.Tensor:
-    """Product-state factors with the leading rotations applied (leading_rotations):
-    their unitaries are composed per bit position on the device (one recipe
-    per position, per sample when an input angle is among them) and applied by
-    qs_factor_apply."""
-    key = ("leading_tables", B, str(efac.device))
+def _leading_tables(prog: Program, per_pos, B: int, device):
+    """Recipes of the leading rotations' per-position unitaries (absorb_factors),
+    cached per batch and device."""
+    key = ("leading_tables", B, str(device))
     hit = prog.words_cache.get(key)
     if hit is None:
         index, per_sample, _, _ = coefficient_layout(prog, B)
@@ -728,11 +734,19 @@ def absorb_factors(prog: Program, per_pos, efac: torch.Tensor, cs: torch.Tensor,
         for pos in sorted(per_pos):
             u, ps, _ = tables.add([(g.kind, index[g.slot], per_sample[g.slot]) for g in per_pos[pos]])
             unit[pos], span[pos] = u, (B if ps else 1)
-        recipes = torch.from_numpy(tables.recipes()).to(efac.device)
-        offsets = torch.tensor(tables.offsets, dtype=torch.int32, device=efac.device)
+        recipes = L.upload(tables.recipes(), torch.int32, device)
+        offsets = L.upload(tables.offsets, torch.int32, device)
         hit = (recipes, offsets, tables.count, tables.nu, unit, span)
         prog.words_cache[key] = hit
-    recipes, offsets, nrec, nu, unit, span = hit
+    return hit
+
+
+def absorb_factors(prog: Program, per_pos, efac: torch.Tensor, cs: torch.Tensor, B: int) -> torch.Tensor:
+    """Product-state factors with the leading rotations applied (leading_rotations):
+    their unitaries are composed per bit position on the device (one recipe
+    per position, per sample when an input angle is among them) and applied by
+    qs_factor_apply."""
+    recipes, offsets, nrec, nu, unit, span = _leading_tables(prog, per_pos, B, efac.device)
     lib = L.load()
     st = L.stream_of(efac)
     utab = torch.empty(max(nu, 1) * 8, dtype=torch.float32, device=efac.device)
@@ -771,7 +785,7 @@ def specialised_kernels(prog: Program, pl: PL.Plan, ws: WordSet, B: int, gen_fir
     hit = prog.words_cache.get(("jit",) + key)
     if hit is None:
         masks = readout_fold(prog)[1] if fold else None
-        hit = jit.ProgramKernels(ws.passes, prog.n, pl.R, pl.TB, gen_first, prog.measure, readout=masks,
+        hit = jit.ProgramKernels(ws.host_passes(), prog.n, pl.R, pl.TB, gen_first, prog.measure, readout=masks,
                                  load=load, **opts)
         if load:
             prog.words_cache[("jit",) + key] = hit
@@ -798,6 +812,34 @@ def forward_kernels(prog: Program, B: int, R: int, TB: int, cm: PL.CostModel = N
     pl = get_plan(prog, R, TB, cm, fold)
     ws = pass_words(prog, pl, B, device)
     return specialised_kernels(prog, pl, ws, B, True, fold, load=load)
+
+
+def warm(prog: Program, B: int, R: int, TB: int, cm: PL.CostModel = None, specialise: bool = None,
+         device="cuda", want_state: bool = False, input_state: bool = False):
+    """Builds, uploads and compiles what execute needs for this program and
+    batch (the leading rotations' tables, the plan, the pass words and
+    fused-unitary recipes, the compiled kernels and their lane-table index)
+    without launching any work.  A caller that will launch on a busy stream
+    (a serving loop) warms ahead, so that execute then does no host work
+    that waits for the device.  ``input_state``: the call passes an initial
+    state (no product-state factors)."""
+    from .. import config
+
+    n = prog.n
+    jit_on = use_jit(n, R, TB) if specialise is None else specialise
+    product = not (prog.init[0] in ("input", "amplitude") and input_state)
+    if product and getattr(config, "ABSORB_LEADING_ROTATIONS", True):
+        lead = leading_rotations(prog)
+        if lead is not None:
+            _leading_tables(prog, lead[1], B, device)
+            prog = lead[0]
+    fold = folds(prog, jit_on) and not want_state
+    pl = get_plan(prog, R, TB, cm, fold)
+    ws = pass_words(prog, pl, B, device)
+    if jit_on:
+        kern = specialised_kernels(prog, pl, ws, B, product, fold)
+        if kern.lanes == "lds":
+            kern.lane_index_on(device)
 
 
 def execute(prog: Program, inp: Inputs, cs: torch.Tensor, efac, R: int, TB: int,

@@ -1687,7 +1687,7 @@ class ProgramKernels:
         if t is None:
             import torch
 
-            t = torch.tensor(self.lane_index, dtype=torch.int32, device=device)
+            t = L.upload(self.lane_index, torch.int32, device)
             self._lane_idx[key] = t
         return t
 
@@ -1759,7 +1759,7 @@ class AdjointKernels:
         if key not in cache:
             import torch
 
-            cache[key] = torch.tensor(self.lane_index, dtype=torch.int32, device=device)
+            cache[key] = L.upload(self.lane_index, torch.int32, device)
         return cache[key]
 
     def launch(self, i: int, psi, lam, utab, qtab, cs, gpart, seed_w, B: int, NS: int, ntiles: int, stream,

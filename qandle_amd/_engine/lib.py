@@ -138,6 +138,41 @@ def ptr(t: torch.Tensor) -> int:
     return t.data_ptr()
 
 
+_copy_streams: dict = {}
+
+
+def upload(values, dtype, device) -> torch.Tensor:
+    """Host values (a list or numpy array) as a tensor on ``device``, resident
+    when this returns.
+
+    A copy from pageable memory on the current stream returns only once every
+    kernel queued before it on that stream has run: a serving loop that lowers
+    a new circuit while its stream is busy would stall on it.  Here the bytes
+    go through pinned host memory on a stream of their own (default priority,
+    nothing else queued on it), and the host waits for that copy alone.  The
+    tensor is allocated on the copy stream and recorded on the current one,
+    so its memory is not reused before the current stream's work at its
+    release has run."""
+    t = torch.as_tensor(values, dtype=dtype)
+    dev = torch.device(device)
+    if dev.type != "cuda":
+        return t
+    if dev.index is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+    with _lock:
+        st = _copy_streams.get(dev.index)
+        if st is None:
+            st = _copy_streams[dev.index] = torch.cuda.Stream(dev)
+    src = t.pin_memory()
+    cur = torch.cuda.current_stream(dev)
+    with torch.cuda.stream(st):
+        out = torch.empty(t.shape, dtype=dtype, device=dev)
+        out.copy_(src, non_blocking=True)
+    out.record_stream(cur)
+    st.synchronize()
+    return out
+
+
 def i32_array(values):
     values = [int(v) for v in values]
     return (ctypes.c_int32 * max(1, len(values)))(*values)

@@ -175,8 +175,10 @@ def precompile(requests, compiled_from_qubits: int, workers: int = 8) -> int:
 class ServingEngine:
     def __init__(self, device, cap_bytes: float = 200e9, streams: int = 4, max_inflight: int = 64,
                  max_wait_s: float = 2.0, compiled_from_qubits: typing.Optional[int] = None,
-                 max_chunk_frac: float = 0.25, small_work: float = 1e11, big_share: float = 0.9):
+                 max_chunk_frac: float = 0.25, small_work: float = 1e11, big_share: float = 0.9,
+                 warm: bool = True):
         self.device = torch.device(device)
+        self.warm = warm
         self.max_wait_s = max_wait_s
         self.cap = int(cap_bytes)
         self.compiled_from = compiled_from_qubits
@@ -197,8 +199,11 @@ class ServingEngine:
 
     # ----------------------------------------------------------- admission
     def prepare(self, req: Request):
-        """Lower + plan (and, for compiled requests, load their kernels) before
-        serving; split over-cap batches into chunks."""
+        """Lower, plan, encode and upload (and, for compiled requests, load their
+        kernels) before serving; split over-cap batches into chunks.  Everything
+        a chunk's launch needs from the host is built here, for every chunk
+        size, so that a launch while the streams are busy never waits for them
+        (an upload or a copy back would wait behind the queued kernels)."""
         prog = fused.lower_circuit(req.circuit.circuit)
         if prog is None or not prog.measure:
             raise ValueError("serving takes circuits that end in MeasureProbability")
@@ -207,10 +212,6 @@ class ServingEngine:
         req.B = 1 if x is None or x.dim() == 1 else x.shape[0]
         req.compiled = self.compiled_from is not None and prog.n >= self.compiled_from
         R, TB = kernel_shape(prog.n, req.compiled)
-        if req.compiled:
-            fused.forward_kernels(prog, 1, R, TB, fused.DEFAULT_COST, self.device)
-        else:
-            fused.get_plan(prog, R, TB, fused.INTERPRETED_COST)
         per = state_footprint(req.n, 1, R, TB)
         req.footprint = state_footprint(req.n, req.B, R, TB)
         req.work = (len(prog.gates) + prog.n) << prog.n  # amplitude updates per sample
@@ -222,6 +223,11 @@ class ServingEngine:
         while step > 1 and state_footprint(req.n, step, R, TB) > limit:
             step -= 1
         req.chunks = [(lo, min(req.B, lo + step)) for lo in range(0, req.B, step)]
+        if self.warm:
+            cm = fused.DEFAULT_COST if req.compiled else fused.INTERPRETED_COST
+            for b in sorted({hi - lo for lo, hi in req.chunks}):
+                fused.warm(prog, b, R, TB, cm, req.compiled, self.device)
+            fused.param_angles(prog, self.device)
         return req
 
     def submit(self, req: Request):

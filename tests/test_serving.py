@@ -91,6 +91,7 @@ def _host_engine(cap, small_work=-1):
 
     eng = serving.ServingEngine.__new__(serving.ServingEngine)
     eng.cap, eng.max_inflight, eng.max_wait_s, eng.compiled_from = int(cap), 64, 2.0, None
+    eng.warm = False  # no device: nothing is encoded or uploaded ahead
     eng.max_chunk_frac, eng.small_work, eng.big_cap = 0.25, small_work, int(cap * 0.9)
     eng.stream_work = [0] * 4
     eng.queue, eng.inflight, eng.done = [], [], []
