@@ -98,6 +98,12 @@ class _Chunk:
     keep: typing.Any = None
     work: int = 0       # amplitude updates: (gates + qubits) x samples x 2^n
     stream: int = -1    # index of the stream it was enqueued on
+    region: typing.Any = None  # (offset, bytes) of its state in the engine's arena
+
+    @property
+    def state_bytes(self) -> int:
+        """Bytes of its (samples, 2^n) complex64 state."""
+        return ((self.hi - self.lo) << self.req.n) * 8
 
 
 def state_footprint(n: int, B: int, R: int, TB: int) -> int:
@@ -106,6 +112,69 @@ def state_footprint(n: int, B: int, R: int, TB: int) -> int:
     lib = L.load()
     part = max(lib.qs_fused_measure_work_bytes(B, n, R, TB), lib.qs_readout_work_bytes(B, n, R, TB), 0)
     return B * (2 ** n) * 8 + part + B * n * 32 + 4096
+
+
+class StateArena:
+    """One device allocation that the serving loop carves its chunks' states
+    from, so that taking and returning a state is host bookkeeping.  With
+    states of KBs to tens of GB coming and going, a caching allocator that
+    cannot place a large one frees its cached blocks first, which waits for
+    every stream; the loop then stopped (no admissions, no completions) for as
+    long as the largest kernels in flight ran.  Regions are aligned to 64 KiB;
+    small states are placed from the top end and large ones from the bottom,
+    so that small ones do not split the space large ones need."""
+
+    ALIGN = 1 << 16
+
+    def __init__(self, nbytes: int, device):
+        self.size = max(int(nbytes) // self.ALIGN * self.ALIGN, self.ALIGN)
+        self.buf = torch.empty(self.size // 8, dtype=torch.complex64, device=device)
+        self.free = [(0, self.size)]  # (offset, length), sorted by offset, neighbours coalesced
+
+    def _span(self, nbytes: int) -> int:
+        return -(-int(nbytes) // self.ALIGN) * self.ALIGN
+
+    def fits(self, nbytes: int) -> bool:
+        need = self._span(nbytes)
+        return any(ln >= need for _, ln in self.free)
+
+    def take(self, nbytes: int, top: bool = False):
+        """Offset of a region of ``nbytes`` (None when no free extent holds it):
+        the lowest extent's start, or with ``top`` the highest extent's end."""
+        need = self._span(nbytes)
+        order = range(len(self.free) - 1, -1, -1) if top else range(len(self.free))
+        for i in order:
+            off, ln = self.free[i]
+            if ln < need:
+                continue
+            if top:
+                self.free[i] = (off, ln - need)
+                got = off + ln - need
+            else:
+                self.free[i] = (off + need, ln - need)
+                got = off
+            if self.free[i][1] == 0:
+                del self.free[i]
+            return got
+        return None
+
+    def give(self, off: int, nbytes: int):
+        """Returns a region taken with the same size."""
+        need = self._span(nbytes)
+        lo = 0
+        while lo < len(self.free) and self.free[lo][0] < off:
+            lo += 1
+        self.free.insert(lo, (off, need))
+        if lo + 1 < len(self.free) and off + need == self.free[lo + 1][0]:
+            self.free[lo] = (off, need + self.free[lo + 1][1])
+            del self.free[lo + 1]
+        if lo > 0 and self.free[lo - 1][0] + self.free[lo - 1][1] == off:
+            self.free[lo - 1] = (self.free[lo - 1][0], self.free[lo - 1][1] + self.free[lo][1])
+            del self.free[lo]
+
+    def state(self, off: int, B: int, n: int) -> torch.Tensor:
+        """The (B, 2^n) complex64 state at byte offset ``off``."""
+        return self.buf[off // 8: off // 8 + B * (1 << n)].view(B, 1 << n)
 
 
 def kernel_shape(n: int, compiled: bool):
@@ -176,9 +245,11 @@ class ServingEngine:
     def __init__(self, device, cap_bytes: float = 200e9, streams: int = 4, max_inflight: int = 64,
                  max_wait_s: float = 2.0, compiled_from_qubits: typing.Optional[int] = None,
                  max_chunk_frac: float = 0.25, small_work: float = 1e11, big_share: float = 0.9,
-                 warm: bool = True):
+                 warm: bool = True, arena: bool = True):
         self.device = torch.device(device)
         self.warm = warm
+        self.use_arena = arena
+        self.arena = None
         self.max_wait_s = max_wait_s
         self.cap = int(cap_bytes)
         self.compiled_from = compiled_from_qubits
@@ -244,6 +315,16 @@ class ServingEngine:
     def now(self) -> float:
         return time.perf_counter() - self.t0
 
+    def _arena(self):
+        """The state arena (the admission cap's bytes, allocated on first use;
+        serve() drops it at its end, and the caching allocator keeps the block
+        for the next arena of the same size)."""
+        if not getattr(self, "use_arena", False):
+            return None
+        if self.arena is None:
+            self.arena = StateArena(self.cap, self.device)
+        return self.arena
+
     def _order(self):
         """Aged chunks (waited longer than max_wait_s) in arrival order, then the
         rest by footprint (ties by arrival)."""
@@ -262,6 +343,9 @@ class ServingEngine:
         beside the reservation or in the small pool (the cap less the large share,
         which large chunks never take)."""
         if self.resident + ch.bytes > self.cap:
+            return False
+        arena = self._arena()
+        if arena is not None and not arena.fits(ch.state_bytes):
             return False
         if self.small(ch):
             return (self.resident + ch.bytes + reserve <= self.cap
@@ -318,12 +402,36 @@ class ServingEngine:
         ch.stream = self.pick_stream(ch)
         s = self.streams[ch.stream]
         s.wait_stream(torch.cuda.current_stream(self.device))
-        # the state comes from the allocating stream's pool and is recorded on the launch stream:
-        # one pool then serves the chunks of every stream (a block freed by a chunk on one stream is
-        # reused by the next large chunk on any other once its work is done), instead of each stream
-        # caching its own blocks and new ones being mapped for every large chunk
-        psi = torch.empty(ch.hi - ch.lo, 2 ** req.n, dtype=torch.complex64, device=self.device)
-        psi.record_stream(s)
+        arena = self._arena()
+        if arena is not None:
+            off = arena.take(ch.state_bytes, top=self.small(ch))
+            if off is None:  # fits() checked an extent; nothing between took it
+                raise RuntimeError("state arena has no extent for an admitted chunk")
+            ch.region = (off, ch.state_bytes)
+            psi = arena.state(off, ch.hi - ch.lo, req.n)
+        else:
+            # from the allocating stream's pool, recorded on the launch stream: one pool serves
+            # the chunks of every stream instead of each stream caching its own blocks
+            psi = torch.empty(ch.hi - ch.lo, 2 ** req.n, dtype=torch.complex64, device=self.device)
+            psi.record_stream(s)
+        try:
+            self._run(ch, s, psi)
+        except BaseException:
+            if ch.region is not None:
+                s.synchronize()  # kernels enqueued before the failure may still write the region
+            self._give_region(ch)
+            raise
+        if req.t_admit < 0:
+            req.t_admit = self.now()
+        self._hold(ch)
+
+    def _give_region(self, ch: _Chunk):
+        if ch.region is not None and self.arena is not None:
+            self.arena.give(*ch.region)
+        ch.region = None
+
+    def _run(self, ch: _Chunk, s, psi):
+        req = ch.req
         with torch.cuda.stream(s), torch.no_grad():
             kw = dict(req.inputs)
             prog = fused.lower_circuit(req.circuit.circuit)
@@ -342,9 +450,6 @@ class ServingEngine:
             ev = torch.cuda.Event()
             ev.record(s)
             ch.event = ev
-        if req.t_admit < 0:
-            req.t_admit = self.now()
-        self._hold(ch)
 
     def _hold(self, ch: _Chunk):
         """Account an admitted chunk's memory and stream work."""
@@ -357,6 +462,7 @@ class ServingEngine:
         self.inflight.append(ch)
 
     def _release(self, ch: _Chunk):
+        self._give_region(ch)  # its kernels have run (its event completed)
         self.resident -= ch.bytes
         if self.small(ch):
             self.small_resident -= ch.bytes
@@ -406,6 +512,7 @@ class ServingEngine:
                 nxt = pending[i].arrival - self.now() if i < len(pending) else 0.0
                 time.sleep(min(max(nxt, 0.0), 2e-4) if nxt > 0 else 5e-5)
         torch.cuda.synchronize(self.device)
+        self.arena = None
         return self.done
 
 

@@ -253,3 +253,50 @@ def test_precompile_workers_fill_the_code_cache(tmp_path, monkeypatch):
 
     monkeypatch.setattr(jit, "_compile", no_compile)
     assert serving.precompile(reqs, 12, workers=1) == 2
+
+
+def test_state_arena_regions():
+    """The serving arena's host bookkeeping: aligned regions, small ones from the
+    top end and large ones from the bottom, coalescing on return, and no region
+    handed out twice (checked against a byte map)."""
+    import numpy as np
+
+    A = serving.StateArena.__new__(serving.StateArena)
+    A.size, A.free = 64 * serving.StateArena.ALIGN, [(0, 64 * serving.StateArena.ALIGN)]
+    al = serving.StateArena.ALIGN
+    used = np.zeros(64, dtype=bool)
+    rng = np.random.default_rng(0)
+    live = []
+    for step in range(400):
+        if live and (rng.random() < 0.45 or not A.fits(al)):
+            off, nb = live.pop(int(rng.integers(len(live))))
+            A.give(off, nb)
+            used[off // al: off // al + -(-nb // al)] = False
+        else:
+            nb = int(rng.integers(1, 9 * al))
+            top = bool(rng.random() < 0.5)
+            off = A.take(nb, top=top)
+            if off is None:
+                assert not A.fits(nb)
+                continue
+            blocks = -(-nb // al)
+            assert off % al == 0 and off + blocks * al <= A.size
+            assert not used[off // al: off // al + blocks].any()
+            used[off // al: off // al + blocks] = True
+            live.append((off, nb))
+        # free list: sorted, disjoint, coalesced, and exactly the unused blocks
+        offs = [o for o, _ in A.free]
+        assert offs == sorted(offs)
+        for (o1, l1), (o2, _l2) in zip(A.free, A.free[1:]):
+            assert o1 + l1 < o2
+        free_map = np.zeros(64, dtype=bool)
+        for o, ln in A.free:
+            free_map[o // al: (o + ln) // al] = True
+        assert (free_map == ~used).all()
+    for off, nb in live:
+        A.give(off, nb)
+    assert A.free == [(0, A.size)]
+    # small from the top, large from the bottom
+    a = A.take(al, top=True)
+    b = A.take(2 * al)
+    assert a == A.size - al and b == 0
