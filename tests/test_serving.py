@@ -300,3 +300,37 @@ def test_state_arena_regions():
     a = A.take(al, top=True)
     b = A.take(2 * al)
     assert a == A.size - al and b == 0
+
+
+@pytest.mark.gpu
+def test_prepared_launches_upload_nothing(device, monkeypatch):
+    """After prepare, a chunk's launch does no host-to-device upload (an upload
+    would wait behind the kernels queued on a busy stream): every plan, word
+    set, table and kernel it needs was built for its chunk size ahead.  The
+    requests are chunked (a cap that holds two samples of the largest), mix
+    compiled and interpreting kernels, and match the same requests served
+    without the arena."""
+    from qandle_amd._engine import lib as L
+
+    spec = [(11, 5, 2, 1), (13, 6, 3, 2), (15, 4, 2, 3)]
+    per = serving.state_footprint(15, 1, *serving.kernel_shape(15, True))
+    reqs = [_req(i, n, B, d, s, device)[0] for i, (n, B, d, s) in enumerate(spec)]
+    plain = [_req(i, n, B, d, s, device)[0] for i, (n, B, d, s) in enumerate(spec)]
+    eng = serving.ServingEngine(device, cap_bytes=9 * per, streams=2, compiled_from_qubits=13)
+    for r in reqs:
+        eng.prepare(r)
+    assert any(len(r.chunks) > 1 for r in reqs) and {r.compiled for r in reqs} == {True, False}
+
+    def no_upload(*a, **k):
+        raise AssertionError("a prepared launch uploaded host data")
+
+    monkeypatch.setattr(L, "upload", no_upload)
+    done = eng.serve(reqs)
+    monkeypatch.undo()
+    assert all(r.error is None for r in done)
+    ref = serving.ServingEngine(device, cap_bytes=9 * per, streams=2, compiled_from_qubits=13, arena=False)
+    for r in plain:
+        ref.prepare(r)
+    ref.serve(plain)
+    for a, b in zip(reqs, plain):
+        assert torch.equal(a.result, b.result)
