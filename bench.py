@@ -329,7 +329,7 @@ def hbm_roofline(events, kernel):
 
 def cpu_share() -> int:
     """Host CPUs this process may use: its affinity set, capped by OMP_NUM_THREADS
-    when the environment sets it (the box's per-GPU CPU share)."""
+    when the environment sets it (a host's CPU share per GPU)."""
     n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     env = os.environ.get("OMP_NUM_THREADS", "")
     if env.isdigit() and int(env) > 0:

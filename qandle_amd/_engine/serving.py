@@ -248,7 +248,7 @@ _STREAMS: dict = {}
 def serving_streams(device, count: int):
     """The serving streams of a device: created once per process and shared by
     every engine on it.  HIP maps streams onto a fixed number of hardware
-    queues (GPU_MAX_HW_QUEUES, 4 on the boxes measured here), and a kernel
+    queues (GPU_MAX_HW_QUEUES, 4 on the machines measured), and a kernel
     waits in order behind every kernel queued before it on its hardware
     queue.  Engines that each took new streams from torch's stream pool put
     the small-request stream on a queue shared with a large-chunk stream often

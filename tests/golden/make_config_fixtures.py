@@ -2,7 +2,7 @@
 
     python tests/golden/make_config_fixtures.py [c2 c3 c4 c5]
 
-The GPU box only compares against these files: every expected value here comes
+The GPU machine only compares against these files: every expected value here comes
 from the FP64 oracle (oracle/fp64.py, itself pinned by tests/test_oracle.py),
 computed in the build container.  Each file also stores the inputs and angles
 it was computed from, so a GPU test can check that the seeded workload

@@ -1,7 +1,7 @@
 """GPU parity at the benchmark workloads' full sizes, against FP64 fixtures.
 
 The FP64 values come from tests/golden/make_config_fixtures.py (run in the
-build container; the box only compares).  Each test first checks that the
+build container; the GPU machine only compares).  Each test first checks that the
 seeded generators still produce the circuit the fixture was computed for.
 
 Bars: probabilities 1e-5 absolute, gradients 1e-4 absolute (the FP64 error of
