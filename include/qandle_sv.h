@@ -316,9 +316,11 @@ int qs_adjoint_finish(float *out, float *tot, const float *gpart, int rows,
  * device memory.  `items` (device) holds int32 pairs (descriptor index, tile
  * index); one launch runs every item with a grid-stride loop over `grid`
  * workgroups, so circuits of different widths, depths and programs share one
- * launch and can be captured into a single hipGraph.  Replaces running the
- * reference's per-circuit Python executor (src/qandle/qcircuit.py:163-174)
- * once per circuit.
+ * launch and can be captured into a single hipGraph.  The item launches take
+ * the tile shapes (R, TB) = (4, 7), (4, 8), (5, 6) and (5, 8) and return
+ * QS_EUNSUPPORTED for any other.  Replaces running the reference's
+ * per-circuit Python executor (src/qandle/qcircuit.py:163-174) once per
+ * circuit.
  * ------------------------------------------------------------------------- */
 /* Product-state factors of many circuits in one launch: entry e writes
  * efac[e] = (f(0), f(1)) (two float2) from x[entries[2e]] with rotation

@@ -129,7 +129,7 @@ def tune(prog, inp, reps: int = 3, cands=None):
     return best, table
 
 
-def tune_uneven(make_batch, cands=((4, 8, 1024.0), (4, 8, 2048.0), (4, 7, 1024.0), (5, 6, 1024.0)), reps=3,
+def tune_uneven(make_batch, cands=((4, 8, 1024.0), (4, 8, 2048.0), (4, 7, 1024.0), (5, 8, 1024.0), (5, 6, 1024.0)), reps=3,
                 backward=True):
     """Build an uneven batch per candidate (via ``make_batch(R, TB, cost)``), time a
     captured replay of each, return (best batch, table)."""

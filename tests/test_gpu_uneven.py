@@ -18,7 +18,7 @@ def make(num, n_lo, n_hi, d_lo, d_hi, seed, device):
     return ws, circs
 
 
-@pytest.mark.parametrize("R,TB", [(4, 7), (4, 8), (5, 6)])
+@pytest.mark.parametrize("R,TB", [(4, 7), (4, 8), (5, 6), (5, 8)])
 def test_uneven_forward_backward_vs_single(device, R, TB):
     ws, circs = make(40, max(4, R), 14, 2, 8, 21, device)
     ub = UnevenBatch(circs, [w.inputs["x"] for w in ws], device, R=R, TB=TB, grid=512)
