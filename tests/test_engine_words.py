@@ -204,3 +204,34 @@ def test_uneven_batch_words_in_bounds_and_match_fp64():
         np.testing.assert_allclose(got, [gref[i] for i in rot_k], atol=1e-6, err_msg=f"circuit {ci}")
         gx = np.stack([grads[c.nrot + j] for j in range(c.d)], 1)
         np.testing.assert_allclose(gx, gref[0], atol=1e-6, err_msg=f"circuit {ci}")
+
+
+def test_warm_builds_what_execute_reads(monkeypatch):
+    """fused.warm encodes and uploads, for one program and batch, everything a
+    forward reads from the host (the leading rotations' tables, the plan, the
+    pass words and table recipes); warming again uploads nothing, and the
+    words are kept on the host for the kernel compilers.  CPU tensors stand in
+    for the device ones here."""
+    from qandle_amd import workloads
+    from qandle_amd._engine import lib as L
+
+    w = workloads.rxryrz_ladder(n=9, B=3, depth=3, seed=4)
+    prog = fused.lower_circuit(qandle.Circuit(w.layers, num_qubits=9).circuit)
+    R, TB = fused.interpreted_shape(9)
+    calls = []
+    real = L.upload
+    monkeypatch.setattr(L, "upload", lambda *a: calls.append(a) or real(*a))
+    fused.warm(prog, 3, R, TB, fused.INTERPRETED_COST, False, "cpu")
+    first = len(calls)
+    assert first > 0
+    fused.warm(prog, 3, R, TB, fused.INTERPRETED_COST, False, "cpu")
+    assert len(calls) == first
+    lead = fused.leading_rotations(prog)
+    assert lead is not None and ("leading_tables", 3, "cpu") in prog.words_cache
+    rest = lead[0]
+    pl = fused.get_plan(rest, R, TB, fused.INTERPRETED_COST, False)
+    ws = fused.pass_words(rest, pl, 3, "cpu")
+    assert len(calls) == first
+    assert [h[0].tolist() for h in ws.host_passes()] == [d[0].tolist() for d in ws.passes]
+    up = L.upload([1, 2, 3], torch.int32, "cpu")
+    assert up.device.type == "cpu" and up.dtype == torch.int32 and up.tolist() == [1, 2, 3]
