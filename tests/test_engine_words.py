@@ -147,16 +147,19 @@ def test_engine_words_in_bounds_and_match_fp64(name):
     np.testing.assert_allclose(gx, gref[0], atol=1e-9)
 
 
-def test_uneven_batch_words_in_bounds_and_match_fp64():
+@pytest.mark.parametrize("R,TB", [(4, 7), (4, 8), (5, 8)])
+def test_uneven_batch_words_in_bounds_and_match_fp64(R, TB):
     """UnevenBatch encodes every circuit against one shared coefficient table and
     one shared pair of fused-unitary tables (absolute indices): each circuit's
-    words, executed alone against those shared tables, reproduce its oracle."""
+    words, executed alone against those shared tables, reproduce its oracle
+    (at each tile shape the item kernels take: tiles of several samples and
+    tiles inside one sample)."""
     from qandle_amd import workloads
     from qandle_amd._engine.uneven import UnevenBatch
 
     wls = [workloads.random_uneven(1, n_lo=n, n_hi=n, d_lo=3, d_hi=3, seed=s, build_layers=False)[0]
            for n, s in ((5, 1), (12, 2), (9, 3), (14, 4))]
-    ub = UnevenBatch.from_workloads(wls, "cpu")
+    ub = UnevenBatch.from_workloads(wls, "cpu", R=R, TB=TB)
     ub._build_backward()
     R, TB = ub.R, ub.TB
     phi = ub.phi.numpy().astype(np.float64)
